@@ -1,0 +1,231 @@
+/*
+ * segscan.h — C ABI of the MI355X (gfx950) SpMM / segment_coo library.
+ *
+ * Plain C: pointers, sizes and enums only. Every device pointer is a HIP device
+ * address (hipMalloc / torch CUDA tensor); `stream` is a hipStream_t passed as
+ * void* (NULL = the default stream). All calls are stream-ordered and
+ * asynchronous; they never synchronise the device and never allocate memory:
+ * scratch space comes in through an explicit `workspace` argument whose size is
+ * reported by the matching *_workspace_size call.
+ *
+ * Arguments are validated on the host before any HIP call; a bad argument
+ * returns SEGSCAN_EINVAL / SEGSCAN_EUNSUPPORTED with a message available from
+ * segscan_last_error() (thread-local), and nothing is launched.
+ *
+ * Reference interfaces replaced (zhang677/segScan):
+ *   segscan_spmm_csr     <- the eight templated SpMM launchers
+ *                           test-cg/include/spmm/spmm.cuh:167-202 (ebpr), :310-343 (rbpr),
+ *                           :522-591 (ebsr), :621-637 (rbsr) and
+ *                           test-cg/include/spmm/spmm_cg.cuh:176-218 (ebpr_cg),
+ *                           :333-374 (rbpr_cg), :553-591 (ebsr_cg), :664-692 (rbsr_cg)
+ *   segscan_segment_coo  <- the six segment_coo launchers
+ *                           new-cg/include/segscan/segscan.cuh:150-184 (segment_coo),
+ *                           :327-355 (segment_coo_new), :817-866 (segment_coo_sr),
+ *                           :868-916 (segment_coo_noshmem_sr),
+ *                           :918-972 (segment_coo_noshmem_lessatom_sr),
+ *                           :974-1008 (segment_coo_sorted_sr)
+ *   segscan_csr_transpose <- the host transpose test-cg/include/dataloader/dataloader.hpp:123-144
+ *                           (+ compressedRow :109-121), done on the device
+ *   segscan_sddmm_csr, segscan_segment_gather, segscan_segment_counts: backward
+ *                           pieces with no reference counterpart (the reference has
+ *                           no backward); they serve the gradient of the two ops above.
+ */
+#ifndef SEGSCAN_H_
+#define SEGSCAN_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum segscan_status {
+  SEGSCAN_OK = 0,
+  SEGSCAN_EINVAL = 1,        /* null pointer, negative size, bad stride, misaligned operand */
+  SEGSCAN_EUNSUPPORTED = 2,  /* knob combination outside the compiled set */
+  SEGSCAN_EHIP = 3,          /* the HIP runtime reported an error (no device, launch failure) */
+  SEGSCAN_EWORKSPACE = 4     /* workspace missing or smaller than *_workspace_size reported */
+} segscan_status_t;
+
+/* Storage type of the dense operand (B for SpMM, src for segment_coo).
+ * Arithmetic is always fp32; outputs are always fp32. */
+typedef enum segscan_dtype {
+  SEGSCAN_F32 = 0,
+  SEGSCAN_BF16 = 1,
+  SEGSCAN_F16 = 2
+} segscan_dtype_t;
+
+/* SpMM variants. EB = element (nnz) balanced, RB = row balanced,
+ * PR = parallel (cross-lane) reduction, SR = sequential (per-lane) reduction.
+ * The *_CG variants take every knob from segscan_knobs_t (spmm_cg.cuh); the
+ * PLAIN_* variants ignore the knobs and use the fixed configuration of the
+ * plain launchers in spmm.cuh. AUTO picks variant and knobs from the
+ * autotuner table (or a built-in heuristic when no table entry matches). */
+typedef enum segscan_spmm_variant {
+  SEGSCAN_SPMM_EBPR = 0,        /* csrspmm_parreduce_nnzbalance_cg */
+  SEGSCAN_SPMM_RBPR = 1,        /* csrspmm_parreduce_rowbalance_cg */
+  SEGSCAN_SPMM_EBSR = 2,        /* csrspmm_rowcaching_nnzbalance_cg */
+  SEGSCAN_SPMM_RBSR = 3,        /* csrspmm_seqreduce_rowbalance_cg */
+  SEGSCAN_SPMM_PLAIN_EBPR = 4,  /* csrspmm_parreduce_nnzbalance */
+  SEGSCAN_SPMM_PLAIN_RBPR = 5,  /* csrspmm_parreduce_rowbalance */
+  SEGSCAN_SPMM_PLAIN_EBSR = 6,  /* csrspmm_rowcaching_nnzbalance */
+  SEGSCAN_SPMM_PLAIN_RBSR = 7,  /* csrspmm_seqreduce_rowbalance */
+  SEGSCAN_SPMM_AUTO = 8
+} segscan_spmm_variant_t;
+
+/* segment_coo variants (new-cg/include/segscan/segscan.cuh). */
+typedef enum segscan_seg_variant {
+  SEGSCAN_SEG_PR = 0,               /* segment_coo */
+  SEGSCAN_SEG_PR_NEW = 1,           /* segment_coo_new */
+  SEGSCAN_SEG_SR = 2,               /* segment_coo_sr */
+  SEGSCAN_SEG_NOSHMEM_SR = 3,       /* segment_coo_noshmem_sr */
+  SEGSCAN_SEG_LESSATOM_SR = 4,      /* segment_coo_noshmem_lessatom_sr */
+  SEGSCAN_SEG_SORTED_SR = 5,        /* segment_coo_sorted_sr */
+  SEGSCAN_SEG_AUTO = 6
+} segscan_seg_variant_t;
+
+/* Write policy of the sequential-reduction kernels. */
+typedef enum segscan_write_policy {
+  SEGSCAN_WRITE_DEFAULT = 0,  /* the variant's own policy */
+  SEGSCAN_WRITE_ATOMIC = 1,   /* every flushed row is an atomic add (output zero-filled first) */
+  SEGSCAN_WRITE_OWNER = 2,    /* rows are stored once by the chunk that owns them (empty rows
+                                 zero-filled by their owner); a row that crosses a chunk
+                                 boundary gets a carry added by a fix-up pass; needs workspace */
+  SEGSCAN_WRITE_LESSATOM = 3  /* rows that start and end inside a chunk are plain stores, the
+                                 chunk's first and last row are atomic adds (output zero-filled
+                                 first); the reference's segment_coo_noshmem_lessatom_sr policy */
+} segscan_write_policy_t;
+
+/* Tuning knobs, named as in the reference templates. A field left 0 takes the
+ * variant's default (the reference's best static configuration); since 0 means
+ * "default", a factor of 0 (size 1) is written as -1.
+ *   group_factor     group_size = 1 << group_factor lanes (1..64) that reduce together
+ *                    (PR) or that share one dense row panel (SR)
+ *   tile_factor      tile_size = 1 << tile_factor (1..64): nnz per tile (PR) and the
+ *                    column-panel width per grid slice (PR, RBSR); README "tail_factor"
+ *   thread_per_block lanes per workgroup (rounded up to a multiple of 64, <= 1024)
+ *   block_numer/block_denom  block_factor: workers = rows (or elements) x numer / denom
+ *   coarsen_factor   columns per lane (1, 2, 4, 8); 0 = derived from N as the reference does
+ *   thread_nz        nnz tiles per group per chunk (SR variants)
+ *   dtile_size, dtile_num, block_dim_y   segment_coo_sorted_sr / segment_coo_new shape knobs
+ *   write_policy     segscan_write_policy_t */
+typedef struct segscan_knobs {
+  int32_t group_factor;
+  int32_t tile_factor;
+  int32_t thread_per_block;
+  int32_t block_numer;
+  int32_t block_denom;
+  int32_t coarsen_factor;
+  int32_t thread_nz;
+  int32_t dtile_size;
+  int32_t dtile_num;
+  int32_t block_dim_y;
+  int32_t write_policy;
+} segscan_knobs_t;
+
+/* CSR matrix A (M x K), zero-based, int32 indices, rows sorted by construction.
+ * vals may be NULL: every value is then 1 (reference __guard_load_default_one,
+ * test-cg/include/util/check.cuh:104-111). Mirrors SpMatCsrDescr_t
+ * (test-cg/include/dataloader/dataloader.hpp:146-168). */
+typedef struct segscan_csr {
+  int64_t nrow;
+  int64_t ncol;
+  int64_t nnz;
+  const int32_t* rowptr; /* [nrow + 1], device */
+  const int32_t* colind; /* [nnz], device */
+  const float* vals;     /* [nnz], device, or NULL */
+} segscan_csr_t;
+
+/* Flags. Without SEGSCAN_ACCUMULATE the call computes C = A.B (resp. dst =
+ * segment sums): the library writes every output element, so callers need not
+ * zero C (the reference requires a pre-zeroed C, check.cuh:71). With it the
+ * call computes C += A.B. */
+#define SEGSCAN_ACCUMULATE 1u
+
+/* Launch geometry a call would use; filled by the *_describe functions (host
+ * only, no HIP call), for tests and for the autotuner's bookkeeping. */
+typedef struct segscan_launch_info {
+  int32_t variant;          /* resolved variant (never AUTO) */
+  int32_t group_size;
+  int32_t tile_size;
+  int32_t thread_per_block; /* lanes per workgroup actually launched */
+  int32_t coarsen_factor;
+  int32_t thread_nz;
+  int32_t write_policy;     /* resolved: 1 atomic, 2 owner */
+  int32_t chunk;            /* nnz (or elements) per partition chunk, 0 if none */
+  int64_t grid_x;
+  int64_t grid_y;
+  int64_t nchunks;
+  size_t workspace_bytes;
+} segscan_launch_info_t;
+
+/* ---- SpMM: C[M x N] (=|+=) A[M x K] . B[K x N] -------------------------------------- */
+segscan_status_t segscan_spmm_describe(const segscan_csr_t* A, int64_t N, segscan_dtype_t b_dtype,
+                                       segscan_spmm_variant_t variant, const segscan_knobs_t* knobs,
+                                       segscan_launch_info_t* info);
+segscan_status_t segscan_spmm_workspace_size(const segscan_csr_t* A, int64_t N, segscan_dtype_t b_dtype,
+                                             segscan_spmm_variant_t variant, const segscan_knobs_t* knobs,
+                                             size_t* bytes);
+segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N,
+                                  const void* B, int64_t ldb, segscan_dtype_t b_dtype,
+                                  float* C, int64_t ldc,
+                                  segscan_spmm_variant_t variant, const segscan_knobs_t* knobs,
+                                  uint32_t flags, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- segment_coo: dst[s, :] (=|+=) sum_{e : index[e] == s} src[e, :] ----------------
+ * index: [E] int32, sorted non-decreasing, values in [0, dst_len). src: E x N, row
+ * stride ld_src. dst: dst_len x N fp32, row stride ld_dst. */
+segscan_status_t segscan_segment_coo_describe(int64_t E, int64_t N, int64_t dst_len, segscan_dtype_t src_dtype,
+                                              segscan_seg_variant_t variant, const segscan_knobs_t* knobs,
+                                              segscan_launch_info_t* info);
+segscan_status_t segscan_segment_coo_workspace_size(int64_t E, int64_t N, int64_t dst_len, segscan_dtype_t src_dtype,
+                                                    segscan_seg_variant_t variant, const segscan_knobs_t* knobs,
+                                                    size_t* bytes);
+segscan_status_t segscan_segment_coo(const void* src, int64_t ld_src, segscan_dtype_t src_dtype,
+                                     const int32_t* index, int64_t E, int64_t N,
+                                     float* dst, int64_t ld_dst, int64_t dst_len,
+                                     segscan_seg_variant_t variant, const segscan_knobs_t* knobs,
+                                     uint32_t flags, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- backward pieces ------------------------------------------------------------------
+ * CSR -> CSC of the M x K matrix A: colptr[K+1], rowind[nnz], perm[nnz] with
+ * perm[q] = CSR position of CSC entry q; within a column, entries keep CSR
+ * (row) order, i.e. the result equals a stable sort of the nnz by column. vals_t
+ * may be NULL; otherwise vals_t[q] = vals[perm[q]] (1 when vals is NULL). */
+segscan_status_t segscan_csr_transpose_workspace_size(const segscan_csr_t* A, size_t* bytes);
+segscan_status_t segscan_csr_transpose(const segscan_csr_t* A, int32_t* colptr, int32_t* rowind,
+                                       int32_t* perm, float* vals_t,
+                                       void* workspace, size_t workspace_bytes, void* stream);
+/* SDDMM on A's pattern: out[p] = sum_j X[row(p), j] * Y[colind[p], j]  (p < nnz).
+ * X: M x N (ld_x), Y: K x N (ld_y), both of dtype xy_dtype; out fp32. */
+segscan_status_t segscan_sddmm_csr(const segscan_csr_t* A, int64_t N,
+                                   const void* X, int64_t ld_x, const void* Y, int64_t ld_y,
+                                   segscan_dtype_t xy_dtype, float* out, void* stream);
+/* Gather (backward of segment_coo): dsrc[e, :] = ddst[index[e], :] * scale(index[e]),
+ * scale = 1 / max(counts[s], 1) when counts != NULL (mean pooling), else 1. */
+segscan_status_t segscan_segment_gather(const float* ddst, int64_t ld_ddst, const int32_t* index, int64_t E,
+                                        int64_t N, const int32_t* counts, float* dsrc, int64_t ld_dsrc,
+                                        void* stream);
+/* counts[s] = #{e : index[e] == s}, s < dst_len (sorted index). */
+segscan_status_t segscan_segment_counts(const int32_t* index, int64_t E, int64_t dst_len, int32_t* counts,
+                                        void* stream);
+
+/* ---- device-side input checks (optional; the compute calls trust their inputs) ---------
+ * Writes into *flags_dev (device int32) a bit set: 1 rowptr not monotone or not
+ * spanning [0, nnz], 2 column index out of [0, ncol), 4 columns not strictly increasing in a row. */
+segscan_status_t segscan_check_csr(const segscan_csr_t* A, int32_t* flags_dev, void* stream);
+/* Bit 1: index not sorted, bit 2: index value out of [0, dst_len). */
+segscan_status_t segscan_check_index(const int32_t* index, int64_t E, int64_t dst_len, int32_t* flags_dev,
+                                     void* stream);
+
+/* ---- misc -------------------------------------------------------------------------------- */
+const char* segscan_status_string(segscan_status_t status);
+const char* segscan_last_error(void);
+int32_t segscan_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEGSCAN_H_ */

@@ -1,0 +1,228 @@
+// Device primitives shared by the SpMM and segment_coo kernels (gfx950, wave64).
+//
+//  * typed 32..256-bit loads of a lane's CF contiguous columns, converted to fp32;
+//  * lane shifts within an aligned group of G lanes: DPP row_shl for G <= 16
+//    (one VALU op, no LDS traffic), ds_bpermute (__shfl_down) for G = 32, 64;
+//  * the segmented suffix reduction of the parallel-reduction kernels, built on a
+//    wave ballot of segment ends instead of shuffling row ids every step;
+//  * the wave-cooperative CSR row lookup that replaces a per-lane global binary
+//    search over rowptr with one coalesced 64-entry window load per wave.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace segscan {
+
+constexpr int kWave = 64;
+
+// Storage types of the dense operand. bf16 and f16 are kept as raw bits and
+// widened to fp32 in registers (both widenings are exact).
+struct bf16_t { uint16_t bits; };
+struct f16_t { uint16_t bits; };
+
+__device__ __forceinline__ float to_f32(float x) { return x; }
+__device__ __forceinline__ float to_f32(bf16_t x) { return __uint_as_float(static_cast<uint32_t>(x.bits) << 16); }
+__device__ __forceinline__ float to_f32(f16_t x) { return static_cast<float>(__builtin_bit_cast(_Float16, x.bits)); }
+
+template <int BYTES> struct RawVec;
+template <> struct RawVec<2> { using T = uint16_t; };
+template <> struct RawVec<4> { using T = uint32_t; };
+template <> struct RawVec<8> { using T = uint2; };
+template <> struct RawVec<16> { using T = uint4; };
+
+// Loads CF contiguous elements at p (aligned to CF * sizeof(VT) up to 16 bytes)
+// as one vector load (two for 32 bytes) and widens them to fp32.
+template <typename VT, int CF>
+__device__ __forceinline__ void load_cols(const VT* __restrict__ p, float (&out)[CF]) {
+  constexpr int BYTES = CF * static_cast<int>(sizeof(VT));
+  if constexpr (BYTES <= 16) {
+    using R = typename RawVec<BYTES>::T;
+    R r = *reinterpret_cast<const R*>(p);
+    const VT* e = reinterpret_cast<const VT*>(&r);
+#pragma unroll
+    for (int i = 0; i < CF; ++i) out[i] = to_f32(e[i]);
+  } else {
+    static_assert(BYTES == 32, "at most 32 bytes per lane");
+    uint4 r0 = reinterpret_cast<const uint4*>(p)[0];
+    uint4 r1 = reinterpret_cast<const uint4*>(p)[1];
+    const VT* e0 = reinterpret_cast<const VT*>(&r0);
+    const VT* e1 = reinterpret_cast<const VT*>(&r1);
+#pragma unroll
+    for (int i = 0; i < CF / 2; ++i) {
+      out[i] = to_f32(e0[i]);
+      out[i + CF / 2] = to_f32(e1[i]);
+    }
+  }
+}
+
+// Loads the first nv (< CF) of a lane's columns element by element (residue panel).
+template <typename VT, int CF>
+__device__ __forceinline__ void load_cols_partial(const VT* __restrict__ p, int nv, float (&out)[CF]) {
+#pragma unroll
+  for (int i = 0; i < CF; ++i) out[i] = (i < nv) ? to_f32(p[i]) : 0.0f;
+}
+
+template <typename VT, int CF>
+__device__ __forceinline__ void load_cols_any(const VT* __restrict__ p, int nv, float (&out)[CF]) {
+  if (nv >= CF) load_cols<VT, CF>(p, out);
+  else load_cols_partial<VT, CF>(p, nv, out);
+}
+
+// Stores / accumulates CF fp32 columns (vector store when the panel is full).
+template <int CF>
+__device__ __forceinline__ void store_cols(float* __restrict__ p, int nv, const float (&v)[CF], bool accumulate) {
+  if (nv >= CF && !accumulate) {
+    if constexpr (CF == 1) {
+      p[0] = v[0];
+    } else if constexpr (CF == 2) {
+      *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    } else if constexpr (CF == 4) {
+      *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+      reinterpret_cast<float4*>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+      reinterpret_cast<float4*>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < CF; ++i) {
+    if (i < nv) p[i] = accumulate ? p[i] + v[i] : v[i];
+  }
+}
+
+template <int CF>
+__device__ __forceinline__ void atomic_add_cols(float* __restrict__ p, int nv, const float (&v)[CF]) {
+#pragma unroll
+  for (int i = 0; i < CF; ++i) {
+    if (i < nv) unsafeAtomicAdd(p + i, v[i]);  // coarse-grained memory: hardware fp32 atomic
+  }
+}
+
+template <int CF>
+__device__ __forceinline__ void zero_cols(float* __restrict__ p, int nv) {
+  float z[CF];
+#pragma unroll
+  for (int i = 0; i < CF; ++i) z[i] = 0.0f;
+  store_cols<CF>(p, nv, z, false);
+}
+
+// ---- lane shifts -------------------------------------------------------------------------
+
+// Lane i receives lane i + D of its 16-lane DPP row (0 past the row end).
+template <int D>
+__device__ __forceinline__ float dpp_row_shl(float v) {
+  static_assert(D >= 1 && D <= 15, "row_shl takes 1..15");
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + D, 0xF, 0xF, true));
+}
+
+// Lane i of an aligned group of G lanes receives lane i + D of the same group.
+// Lanes whose source falls outside the group get an unspecified value; callers
+// mask them out.
+template <int G, int D>
+__device__ __forceinline__ float group_shift_down(float v) {
+  if constexpr (G <= 16) {
+    return dpp_row_shl<D>(v);
+  } else {
+    return __shfl_down(v, D, G);
+  }
+}
+
+// Segmented suffix sum within aligned groups of G lanes. Rows are non-decreasing
+// inside a group. On return the first lane of each run of equal rows (a segment
+// head, `is_head`) holds the run's total in c[]. The segment end of every lane
+// comes from one wave ballot, so each of the log2(G) steps shifts values only.
+template <int G, int CF>
+__device__ __forceinline__ bool group_segmented_reduce(float (&c)[CF], int row) {
+  const int lane = __lane_id();
+  const int g_lane = lane & (G - 1);
+  if constexpr (G == 1) {
+    return true;
+  } else {
+    const int row_next = __shfl_down(row, 1, G);
+    const bool is_end = (g_lane == G - 1) || (row_next != row);
+    const uint64_t ends = __ballot(is_end);
+    const bool is_head = (g_lane == 0) || ((ends >> (lane - 1)) & 1ull);
+    const int seg_end = lane + __builtin_ctzll(ends >> lane);
+#define SEGSCAN_SEG_STEP(D)                                              \
+    if constexpr ((D) < G) {                                             \
+      _Pragma("unroll") for (int i = 0; i < CF; ++i) {                   \
+        const float t = group_shift_down<G, (D)>(c[i]);                  \
+        if (lane + (D) <= seg_end) c[i] += t;                            \
+      }                                                                  \
+    }
+    SEGSCAN_SEG_STEP(1)
+    SEGSCAN_SEG_STEP(2)
+    SEGSCAN_SEG_STEP(4)
+    SEGSCAN_SEG_STEP(8)
+    SEGSCAN_SEG_STEP(16)
+    SEGSCAN_SEG_STEP(32)
+#undef SEGSCAN_SEG_STEP
+    return is_head;
+  }
+}
+
+// Plain sum within aligned groups of G lanes; lane 0 of the group holds the total.
+template <int G, int CF>
+__device__ __forceinline__ void group_reduce_to_first(float (&c)[CF]) {
+#define SEGSCAN_RED_STEP(D)                                              \
+  if constexpr ((D) < G) {                                               \
+    _Pragma("unroll") for (int i = 0; i < CF; ++i) c[i] += group_shift_down<G, (D)>(c[i]); \
+  }
+  SEGSCAN_RED_STEP(32)
+  SEGSCAN_RED_STEP(16)
+  SEGSCAN_RED_STEP(8)
+  SEGSCAN_RED_STEP(4)
+  SEGSCAN_RED_STEP(2)
+  SEGSCAN_RED_STEP(1)
+#undef SEGSCAN_RED_STEP
+}
+
+// ---- CSR row lookup ---------------------------------------------------------------------
+
+// Row containing nnz position e: the r with rowptr[r] <= e < rowptr[r+1], empty
+// rows skipped; e >= nnz maps to M - 1. Same result as the reference's
+// binary_search_segment_number (test-cg/include/util/check.cuh:88-102).
+__device__ __forceinline__ int row_of_nnz(const int32_t* __restrict__ rowptr, int M, int64_t e) {
+  int lo = 1, hi = M;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (rowptr[mid] <= e) lo = mid + 1;
+    else hi = mid;
+  }
+  return hi - 1;
+}
+
+// Wave-cooperative lookup: every lane of the wave must call it (full EXEC).
+// Precondition: rowptr[r0] <= e for every lane and e < rowptr[M] (callers clamp
+// padding lanes to a valid position). The wave loads rowptr[r0+1 .. r0+64]
+// once (one coalesced 256-byte read) and each lane counts the boundaries <= e
+// with a 7-step binary search over the window held in lanes; lanes whose row
+// lies beyond the window move on to the next 64 entries together.
+__device__ __forceinline__ int wave_row_lookup(const int32_t* __restrict__ rowptr, int M, int r0, int e) {
+  const int lane = __lane_id();
+  int base = r0 + 1;
+  int row = r0;
+  bool done = false;
+  while (true) {
+    const int j = base + lane;
+    const int rp = (j <= M) ? rowptr[j] : INT32_MAX;
+    int pos = 0;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+      const int v = __shfl(rp, pos + s - 1);
+      if (v <= e) pos += s;
+    }
+    const int last = __shfl(rp, 63);
+    if (pos == 63 && last <= e) pos = 64;
+    if (!done && pos < 64) {
+      row = base + pos - 1;
+      done = true;
+    }
+    if (__ballot(!done) == 0) break;
+    base += 64;
+  }
+  return row;
+}
+
+}  // namespace segscan

@@ -1,0 +1,118 @@
+// Launch parameter blocks passed by value to the kernels, and the declarations
+// of the per-family launch functions (defined in the .hip files, called by the
+// host dispatcher in segscan_abi.cpp).
+#pragma once
+
+#include <stdint.h>
+#include <stddef.h>
+
+#include <hip/hip_runtime.h>
+
+namespace segscan {
+
+enum class VType : int { F32 = 0, BF16 = 1, F16 = 2 };
+
+// Parallel-reduction SpMM / segment_coo (ebpr, rbpr, segment_coo, segment_coo_new).
+// Lanes of a workgroup are laid out [Y][P][X]: X lanes over consecutive nnz
+// (the tile), P column panels of CF columns, Y tiles per workgroup.
+struct PrParams {
+  int M;              // rows of A (SpMM) / dst_len (segment)
+  int N;              // dense columns
+  int64_t nnz;        // nnz (SpMM) / E (segment)
+  const int32_t* rowptr;   // SpMM only
+  const int32_t* colind;   // SpMM only
+  const float* vals;       // SpMM only, nullable
+  const int32_t* index;    // segment only
+  const void* B;           // dense operand (B or src)
+  int64_t ldb;
+  float* C;                // output (C or dst)
+  int64_t ldc;
+  int X, P, Y;        // lane layout
+  int G;              // group size (template; kept for checks)
+  int slice;          // columns per blockIdx.y
+  int64_t ntiles;     // tiles of X nnz
+  const int32_t* tile_row;  // row of the first nnz of every tile (ebpr)
+  int row_store;      // rbpr: 1 = one partial per row -> plain store, 0 = atomic partials
+  int accumulate;
+};
+
+// Sequential-reduction SpMM / segment_coo (ebsr, segment_coo_sr family).
+// Groups of G lanes cover G*CF columns; each group walks chunks of `chunk`
+// consecutive nnz in order.
+struct SrParams {
+  int M;
+  int N;
+  int64_t nnz;
+  const int32_t* rowptr;
+  const int32_t* colind;
+  const float* vals;
+  const int32_t* index;
+  const void* B;
+  int64_t ldb;
+  float* C;
+  int64_t ldc;
+  int tpb;            // lanes per workgroup (multiple of 64)
+  int64_t chunk;      // nnz per chunk
+  int64_t nchunks;
+  int64_t total_groups;
+  const int32_t* chunk_row;   // row of the first nnz of each chunk (SpMM)
+  const int32_t* chunk_prev;  // row of the nnz just before each chunk, -1 for chunk 0 (SpMM)
+  int32_t* carry_row;         // [nchunks] owner policy: row of the chunk's carry or -1
+  float* carry_val;           // [nchunks x N] owner policy
+  int policy;                 // 1 atomic, 2 owner store + carry, 3 lessatom
+  int accumulate;
+};
+
+// Row-balanced sequential reduction (rbsr): lanes [rows][column vectors].
+struct RbsrParams {
+  int M;
+  int N;
+  const int32_t* rowptr;
+  const int32_t* colind;
+  const float* vals;
+  const void* B;
+  int64_t ldb;
+  float* C;
+  int64_t ldc;
+  int tpr;            // lanes per row (column vectors per slice)
+  int rows_per_block;
+  int slice;
+  int64_t rows_total; // grid-stride bound (M)
+  int accumulate;
+};
+
+struct Launch {
+  dim3 grid;
+  dim3 block;
+  size_t shmem;
+  hipStream_t stream;
+};
+
+// Each returns hipSuccess or the launch error; `ok` false means the (VT, CF, G)
+// combination is not compiled.
+hipError_t launch_spmm_pr(VType vt, int cf, int g, const PrParams& p, const Launch& l, bool eb, bool& ok);
+hipError_t launch_spmm_sr(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok);
+hipError_t launch_spmm_rbsr(VType vt, int cf, const RbsrParams& p, const Launch& l, bool& ok);
+hipError_t launch_seg_pr(VType vt, int cf, int g, const PrParams& p, const Launch& l, bool& ok);
+hipError_t launch_seg_sr(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok);
+
+// Helpers (misc_kernels.hip).
+hipError_t launch_fill_zero(float* C, int64_t rows, int64_t N, int64_t ldc, hipStream_t s);
+hipError_t launch_partition_rows(const int32_t* rowptr, int M, int64_t nnz, int64_t chunk, int64_t nchunks,
+                                 int32_t* chunk_row, int32_t* chunk_prev, hipStream_t s);
+hipError_t launch_carry_fixup(const int32_t* carry_row, const float* carry_val, int64_t nchunks, int N,
+                              float* C, int64_t ldc, hipStream_t s);
+hipError_t launch_transpose(const int32_t* rowptr, const int32_t* colind, const float* vals, int M, int K,
+                            int64_t nnz, int32_t* colptr, int32_t* rowind, int32_t* perm, float* vals_t,
+                            int32_t* counts, int32_t* block_sums, int64_t nblocks, hipStream_t s);
+hipError_t launch_sddmm(VType vt, const int32_t* rowptr, const int32_t* colind, int M, int64_t nnz, int N,
+                        const void* X, int64_t ld_x, const void* Y, int64_t ld_y, float* out, hipStream_t s);
+hipError_t launch_segment_gather(const float* ddst, int64_t ld_ddst, const int32_t* index, int64_t E, int N,
+                                 const int32_t* counts, float* dsrc, int64_t ld_dsrc, hipStream_t s);
+hipError_t launch_segment_counts(const int32_t* index, int64_t E, int64_t dst_len, int32_t* counts,
+                                 hipStream_t s);
+hipError_t launch_check_csr(const int32_t* rowptr, const int32_t* colind, int64_t M, int64_t K, int64_t nnz,
+                            int32_t* flags, hipStream_t s);
+hipError_t launch_check_index(const int32_t* index, int64_t E, int64_t dst_len, int32_t* flags, hipStream_t s);
+
+}  // namespace segscan

@@ -1,0 +1,90 @@
+// Row-balanced sequential reduction (rbsr): one lane computes one row x CF
+// columns, walking the row's nnz in order — the reference's
+// csrspmm_seqreduce_rowbalance (test-cg/include/spmm/spmm.cuh:593-637) and
+// csrspmm_seqreduce_rowbalance_cg (spmm_cg.cuh:593-692).
+//
+// The knobs keep their meaning (tile = columns per grid slice, CF columns per
+// lane, thread_per_block lanes, block_factor rows of work), but lanes are laid
+// out column-fastest ([row][column vector]) so that the lanes of one row read
+// one dense row with consecutive 128-bit loads; the reference's cg variant puts
+// rows fastest, which scatters every load instruction over as many rows as
+// lanes. Values are fp32 (the reference's cg kernel reads them into an int,
+// spmm_cg.cuh:608, which truncates non-integer A).
+#include "device_common.hpp"
+#include "kernel_params.hpp"
+
+namespace segscan {
+
+template <typename VT, int CF>
+__global__ void __launch_bounds__(1024) rbsr_kernel(RbsrParams p) {
+  const int L = threadIdx.x;
+  const int yr = L / p.tpr;
+  const int xv = L - yr * p.tpr;
+  if (yr >= p.rows_per_block) return;
+  const int col0 = blockIdx.y * p.slice + xv * CF;
+  const int nv = min(CF, p.N - col0);
+  if (nv <= 0) return;
+  const VT* __restrict__ B = static_cast<const VT*>(p.B);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * p.rows_per_block;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * p.rows_per_block + yr; r < p.rows_total; r += stride) {
+    const int start = p.rowptr[r];
+    const int end = p.rowptr[r + 1];
+    float acc[CF];
+#pragma unroll
+    for (int i = 0; i < CF; ++i) acc[i] = 0.0f;
+    int q = start;
+    // Two nnz per step so that two dense-row loads are in flight.
+    for (; q + 1 < end; q += 2) {
+      const int64_t k0 = p.colind[q];
+      const int64_t k1 = p.colind[q + 1];
+      const float v0 = p.vals ? p.vals[q] : 1.0f;
+      const float v1 = p.vals ? p.vals[q + 1] : 1.0f;
+      float b0[CF], b1[CF];
+      load_cols_any<VT, CF>(B + k0 * p.ldb + col0, nv, b0);
+      load_cols_any<VT, CF>(B + k1 * p.ldb + col0, nv, b1);
+#pragma unroll
+      for (int i = 0; i < CF; ++i) acc[i] += v0 * b0[i];
+#pragma unroll
+      for (int i = 0; i < CF; ++i) acc[i] += v1 * b1[i];
+    }
+    if (q < end) {
+      const int64_t k0 = p.colind[q];
+      const float v0 = p.vals ? p.vals[q] : 1.0f;
+      float b0[CF];
+      load_cols_any<VT, CF>(B + k0 * p.ldb + col0, nv, b0);
+#pragma unroll
+      for (int i = 0; i < CF; ++i) acc[i] += v0 * b0[i];
+    }
+    store_cols<CF>(p.C + r * p.ldc + col0, nv, acc, p.accumulate != 0);
+  }
+}
+
+template <typename VT, int CF>
+static hipError_t launch_rbsr_t(const RbsrParams& p, const Launch& l) {
+  rbsr_kernel<VT, CF><<<l.grid, l.block, 0, l.stream>>>(p);
+  return hipGetLastError();
+}
+
+template <typename VT>
+static hipError_t launch_rbsr_cf(int cf, const RbsrParams& p, const Launch& l, bool& ok) {
+  ok = true;
+  switch (cf) {
+    case 1: return launch_rbsr_t<VT, 1>(p, l);
+    case 2: return launch_rbsr_t<VT, 2>(p, l);
+    case 4: return launch_rbsr_t<VT, 4>(p, l);
+    case 8: return launch_rbsr_t<VT, 8>(p, l);
+    default: ok = false; return hipSuccess;
+  }
+}
+
+hipError_t launch_spmm_rbsr(VType vt, int cf, const RbsrParams& p, const Launch& l, bool& ok) {
+  switch (vt) {
+    case VType::F32: return launch_rbsr_cf<float>(cf, p, l, ok);
+    case VType::BF16: return launch_rbsr_cf<bf16_t>(cf, p, l, ok);
+    case VType::F16: return launch_rbsr_cf<f16_t>(cf, p, l, ok);
+  }
+  ok = false;
+  return hipSuccess;
+}
+
+}  // namespace segscan

@@ -1,0 +1,955 @@
+// Host side of the C ABI: argument validation, knob resolution (reference
+// defaults, plain-launcher configurations), launch geometry, workspace layout
+// and kernel dispatch. No HIP call is made before every argument has been
+// validated, so every error path here runs without a GPU.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/segscan.h"
+#include "kernel_params.hpp"
+
+namespace {
+
+using namespace segscan;
+
+thread_local std::string g_last_error;
+
+segscan_status_t fail(segscan_status_t st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return st;
+}
+
+segscan_status_t ok() {
+  g_last_error.clear();
+  return SEGSCAN_OK;
+}
+
+segscan_status_t hip_status(hipError_t e, const char* what) {
+  if (e == hipSuccess) return ok();
+  return fail(SEGSCAN_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+constexpr int64_t kMaxI32 = 2147483647LL;
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
+int round_up64(int x) { return (x + 63) / 64 * 64; }
+
+size_t dtype_size(segscan_dtype_t t) { return t == SEGSCAN_F32 ? 4 : 2; }
+bool dtype_valid(int t) { return t == SEGSCAN_F32 || t == SEGSCAN_BF16 || t == SEGSCAN_F16; }
+
+// Elements per 128-bit load.
+int vec_elems(segscan_dtype_t t) { return static_cast<int>(16 / dtype_size(t)); }
+
+// Largest power-of-two column count <= cap that keeps a lane's columns aligned
+// for vector access: divides the leading dimensions and the base addresses.
+int aligned_cf(int cap, const void* B, int64_t ldb, size_t es, const float* C, int64_t ldc) {
+  int cf = 1;
+  for (int c = 2; c <= cap; c <<= 1) {
+    const size_t bvec = std::min<size_t>(es * c, 16);
+    const size_t cvec = std::min<size_t>(4 * c, 16);
+    if (ldb % c) break;
+    if (ldc % c) break;
+    if (B && reinterpret_cast<uintptr_t>(B) % bvec) break;
+    if (C && reinterpret_cast<uintptr_t>(C) % cvec) break;
+    cf = c;
+  }
+  return cf;
+}
+
+// Largest power of two <= cap dividing n (the reference's N % 4 / N % 2 rule).
+int divisor_cf(int64_t n, int cap) {
+  int cf = 1;
+  for (int c = 2; c <= cap; c <<= 1)
+    if (n % c == 0) cf = c;
+  return cf;
+}
+
+struct Knobs {
+  int group_factor = 0, tile_factor = 0, tpb = 0, bn = 0, bd = 0, cf = 0, thread_nz = 0;
+  int dtile_size = 0, dtile_num = 0, block_dim_y = 0, policy = 0;
+};
+
+Knobs from_c(const segscan_knobs_t* k) {
+  Knobs r;
+  if (!k) return r;
+  r.group_factor = k->group_factor;
+  r.tile_factor = k->tile_factor;
+  r.tpb = k->thread_per_block;
+  r.bn = k->block_numer;
+  r.bd = k->block_denom;
+  r.cf = k->coarsen_factor;
+  r.thread_nz = k->thread_nz;
+  r.dtile_size = k->dtile_size;
+  r.dtile_num = k->dtile_num;
+  r.block_dim_y = k->block_dim_y;
+  r.policy = k->write_policy;
+  return r;
+}
+
+// Fills zero fields with `d`.
+void defaults(Knobs& k, const Knobs& d) {
+  if (!k.group_factor) k.group_factor = d.group_factor;
+  if (!k.tile_factor) k.tile_factor = d.tile_factor;
+  if (!k.tpb) k.tpb = d.tpb;
+  if (!k.bn) k.bn = d.bn;
+  if (!k.bd) k.bd = d.bd;
+  if (!k.cf) k.cf = d.cf;
+  if (!k.thread_nz) k.thread_nz = d.thread_nz;
+  if (!k.dtile_size) k.dtile_size = d.dtile_size;
+  if (!k.dtile_num) k.dtile_num = d.dtile_num;
+  if (!k.block_dim_y) k.block_dim_y = d.block_dim_y;
+  if (!k.policy) k.policy = d.policy;
+}
+
+Knobs mk(int gf, int tf, int tpb, int bn, int bd, int cf, int tnz, int policy) {
+  Knobs k;
+  k.group_factor = gf;
+  k.tile_factor = tf;
+  k.tpb = tpb;
+  k.bn = bn;
+  k.bd = bd;
+  k.cf = cf;
+  k.thread_nz = tnz;
+  k.policy = policy;
+  return k;
+}
+
+segscan_status_t check_common_knobs(const Knobs& k) {
+  if (k.tpb < 1 || k.tpb > 1024) return fail(SEGSCAN_EUNSUPPORTED, "thread_per_block %d outside [1, 1024]", k.tpb);
+  if (k.bn < 1 || k.bd < 1) return fail(SEGSCAN_EINVAL, "block_factor %d/%d must be positive", k.bn, k.bd);
+  if (k.thread_nz < 1 || k.thread_nz > (1 << 20))
+    return fail(SEGSCAN_EUNSUPPORTED, "thread_nz %d outside [1, 2^20]", k.thread_nz);
+  return SEGSCAN_OK;
+}
+
+// A knob factor f encodes size 1 << f; the value -1 is accepted as an explicit
+// "0" (size 1), because 0 in the struct means "default".
+int factor_value(int f) { return f < 0 ? 0 : f; }
+
+int64_t workers(int64_t n, const Knobs& k) {
+  // The reference computes (int)((float)n * numer / denom); integer arithmetic here.
+  int64_t w = n * k.bn / k.bd;
+  return std::max<int64_t>(1, w);
+}
+
+// ---------------------------------------------------------------------------------------
+// SpMM planning
+
+enum Family { kEbpr = 0, kRbpr = 1, kEbsr = 2, kRbsr = 3 };
+
+struct SpmmPlan {
+  int variant = 0;
+  Family fam = kEbpr;
+  int G = 1, T = 1, tpb = 64, CF = 1, tnz = 1, policy = 0;
+  int X = 1, P = 1, Y = 1, slice = 1;
+  int64_t grid_x = 1, grid_y = 1;
+  int block = 64;
+  int64_t ntiles = 0, chunk = 0, nchunks = 0, total_groups = 0;
+  int tpr = 1, rows_per_block = 1;
+  bool zero_fill = false;   // output zero-filled before the kernel
+  size_t ws_bytes = 0;
+  size_t off_rows = 0, off_prev = 0, off_crow = 0, off_cval = 0;
+};
+
+// Reference defaults: README.md:168-210 best static tuples for the cg variants;
+// spmm.cuh plain launchers for the plain ones.
+segscan_status_t plan_spmm(int64_t M, int64_t nnz, int64_t N, segscan_dtype_t bt, int variant, const Knobs& user,
+                           int cf_cap, SpmmPlan& pl) {
+  if (variant < 0 || variant > SEGSCAN_SPMM_AUTO) return fail(SEGSCAN_EINVAL, "unknown SpMM variant %d", variant);
+  const int vec = vec_elems(bt);
+  Knobs k = user;
+  int v = variant;
+  if (v == SEGSCAN_SPMM_AUTO) {
+    // Built-in choice for gfx950: element-balanced sequential reduction, a group
+    // of lanes per dense row at 128-bit width, owner stores, 256-nnz chunks.
+    v = SEGSCAN_SPMM_EBSR;
+    int cf = std::min(vec, divisor_cf(N, vec));
+    int lanes = static_cast<int>(std::min<int64_t>(64, ceil_div(N, cf)));
+    int g = 1;
+    while (g < lanes) g <<= 1;
+    int gf = 0;
+    while ((1 << gf) < g) ++gf;
+    k = mk(gf == 0 ? -1 : gf, 5, 256, 1, 1, cf, std::max(1, 256 / g), SEGSCAN_WRITE_OWNER);
+    // enough workgroups for ~8 waves per CU on 256 CUs, the rest grid-strided
+    k.bn = 1;
+    k.bd = std::max<int>(1, static_cast<int>(M / 2048));
+  }
+  pl.variant = v;
+  switch (v) {
+    case SEGSCAN_SPMM_EBPR:
+      defaults(k, mk(5, 5, 256, 1, 16, 0, 1, 0));
+      pl.fam = kEbpr;
+      break;
+    case SEGSCAN_SPMM_RBPR:
+      defaults(k, mk(3, 3, 256, 1, 2, 0, 1, 0));
+      pl.fam = kRbpr;
+      break;
+    case SEGSCAN_SPMM_EBSR:
+      defaults(k, mk(4, 5, 256, 1, 16, 4, 1, 0));
+      pl.fam = kEbsr;
+      break;
+    case SEGSCAN_SPMM_RBSR:
+      defaults(k, mk(0, 5, 256, 1, 1, 4, 1, 0));
+      pl.fam = kRbsr;
+      break;
+    case SEGSCAN_SPMM_PLAIN_EBPR:  // spmm.cuh:167-202: warp 32, 256 threads, M tiles
+      k = mk(5, 5, 256, 1, 1, 0, 1, 0);
+      pl.fam = kEbpr;
+      break;
+    case SEGSCAN_SPMM_PLAIN_RBPR:  // spmm.cuh:310-343
+      k = mk(5, 5, 256, 1, 1, 0, 1, 0);
+      pl.fam = kRbpr;
+      break;
+    case SEGSCAN_SPMM_PLAIN_EBSR: {  // spmm.cuh:522-591: CF by N, ThreadNz 1, CEIL(M, 8) blocks
+      int cf = (N >= 512) ? 4 : (N >= 128) ? 2 : 1;
+      k = mk(5, 5, 256, 1, 8, cf, 1, SEGSCAN_WRITE_ATOMIC);
+      pl.fam = kEbsr;
+      break;
+    }
+    case SEGSCAN_SPMM_PLAIN_RBSR:  // spmm.cuh:621-637: one column per thread, 256-column slices
+      k = mk(0, 8, 256, 1, 1, 1, 1, 0);
+      pl.fam = kRbsr;
+      break;
+    default:
+      return fail(SEGSCAN_EINVAL, "unknown SpMM variant %d", variant);
+  }
+  segscan_status_t st = check_common_knobs(k);
+  if (st) return st;
+  const int gf = factor_value(k.group_factor), tf = factor_value(k.tile_factor);
+  if (gf > 6) return fail(SEGSCAN_EUNSUPPORTED, "group_factor %d: groups are at most 64 lanes (wave64)", gf);
+  if (tf > 8) return fail(SEGSCAN_EUNSUPPORTED, "tile_factor %d too large", tf);
+  pl.G = 1 << gf;
+  pl.T = 1 << tf;
+  if (k.cf && !(k.cf == 1 || k.cf == 2 || k.cf == 4 || k.cf == 8))
+    return fail(SEGSCAN_EUNSUPPORTED, "coarsen_factor %d not in {1,2,4,8}", k.cf);
+  if (k.policy < 0 || k.policy > SEGSCAN_WRITE_LESSATOM)
+    return fail(SEGSCAN_EINVAL, "unknown write policy %d", k.policy);
+  pl.tnz = k.thread_nz;
+
+  switch (pl.fam) {
+    case kEbpr:
+    case kRbpr: {
+      if (pl.T > 64) return fail(SEGSCAN_EUNSUPPORTED, "tile_size %d: tiles are at most one wave (64)", pl.T);
+      if (pl.G > pl.T) return fail(SEGSCAN_EUNSUPPORTED, "group_size %d exceeds tile_size %d", pl.G, pl.T);
+      int cf = k.cf ? k.cf : divisor_cf(N, vec);
+      if (cf == 8 && bt == SEGSCAN_F32) return fail(SEGSCAN_EUNSUPPORTED, "coarsen_factor 8 needs a 16-bit dense operand");
+      cf = std::min(cf, cf_cap);
+      pl.CF = cf;
+      pl.X = pl.T;
+      pl.slice = std::max(pl.T, cf);  // columns per blockIdx.y (reference: tile_size)
+      pl.P = static_cast<int>(ceil_div(std::min<int64_t>(N, pl.slice), cf));
+      if (pl.P * pl.X > 1024) {  // narrower slices instead of an oversized workgroup
+        pl.P = 1024 / pl.X;
+        pl.slice = pl.P * cf;
+      }
+      pl.Y = static_cast<int>(ceil_div(std::max(1, k.tpb / pl.T), pl.P));
+      const int lanes = pl.P * pl.X * pl.Y;
+      if (lanes > 1024)
+        return fail(SEGSCAN_EUNSUPPORTED, "workgroup of %d lanes (panels %d x tile %d x tiles %d) exceeds 1024",
+                    lanes, pl.P, pl.X, pl.Y);
+      pl.block = round_up64(lanes);
+      pl.tpb = pl.block;
+      pl.grid_y = ceil_div(N, pl.slice);
+      if (pl.fam == kEbpr) {
+        pl.ntiles = ceil_div(nnz, pl.T);
+        pl.grid_x = std::min(ceil_div(workers(M, k), pl.Y), std::max<int64_t>(1, ceil_div(pl.ntiles, pl.Y)));
+        pl.zero_fill = true;
+        pl.policy = SEGSCAN_WRITE_ATOMIC;
+        pl.off_rows = 0;
+        pl.ws_bytes = align_up(static_cast<size_t>(std::max<int64_t>(pl.ntiles, 1)) * 4);
+      } else {
+        pl.grid_x = std::min(ceil_div(workers(M, k), pl.Y), std::max<int64_t>(1, ceil_div(M, pl.Y)));
+        const bool store = pl.G == pl.T;
+        pl.zero_fill = !store;
+        pl.policy = store ? SEGSCAN_WRITE_OWNER : SEGSCAN_WRITE_ATOMIC;
+        pl.ws_bytes = 0;
+      }
+      break;
+    }
+    case kEbsr: {
+      if (bt == SEGSCAN_F32 && k.cf == 8 && vec < 8) { /* fp32 CF 8 = two 128-bit loads: allowed */ }
+      // reference: coarsen = min(CEIL(N, group_size), CoarsenFactor) (spmm_cg.cuh:557)
+      int cf = static_cast<int>(std::min<int64_t>(ceil_div(N, pl.G), k.cf ? k.cf : 4));
+      int c2 = 1;
+      while (c2 * 2 <= cf) c2 <<= 1;  // power of two not above it
+      cf = std::min(c2, cf_cap);
+      pl.CF = cf;
+      pl.tpb = round_up64(std::max(k.tpb, pl.G));
+      pl.block = pl.tpb;
+      pl.chunk = static_cast<int64_t>(pl.tnz) * pl.G;
+      pl.nchunks = ceil_div(nnz, pl.chunk);
+      const int gpb = pl.tpb / pl.G;
+      pl.grid_x = std::min(workers(M, k), std::max<int64_t>(1, ceil_div(pl.nchunks, gpb)));
+      pl.total_groups = pl.grid_x * gpb;
+      pl.grid_y = ceil_div(N, static_cast<int64_t>(pl.G) * cf);
+      int policy = k.policy;
+      if (policy == SEGSCAN_WRITE_DEFAULT) policy = (pl.chunk >= 128) ? SEGSCAN_WRITE_OWNER : SEGSCAN_WRITE_LESSATOM;
+      pl.policy = policy;
+      pl.zero_fill = policy != SEGSCAN_WRITE_OWNER;
+      size_t off = 0;
+      pl.off_rows = off;
+      off += align_up(static_cast<size_t>(std::max<int64_t>(pl.nchunks, 1)) * 4);
+      pl.off_prev = off;
+      off += align_up(static_cast<size_t>(std::max<int64_t>(pl.nchunks, 1)) * 4);
+      if (policy == SEGSCAN_WRITE_OWNER) {
+        pl.off_crow = off;
+        off += align_up(static_cast<size_t>(std::max<int64_t>(pl.nchunks, 1)) * 4);
+        pl.off_cval = off;
+        off += align_up(static_cast<size_t>(std::max<int64_t>(pl.nchunks, 1)) * static_cast<size_t>(N) * 4);
+      }
+      pl.ws_bytes = off;
+      break;
+    }
+    case kRbsr: {
+      int cf = k.cf ? k.cf : 1;
+      cf = std::min(cf, cf_cap);
+      pl.CF = cf;
+      pl.slice = std::max(pl.T, cf);
+      pl.tpr = static_cast<int>(ceil_div(std::min<int64_t>(N, pl.slice), cf));
+      pl.rows_per_block = static_cast<int>(std::max<int64_t>(1, ceil_div(k.tpb, pl.tpr)));
+      const int lanes = pl.tpr * pl.rows_per_block;
+      if (lanes > 1024) return fail(SEGSCAN_EUNSUPPORTED, "workgroup of %d lanes exceeds 1024", lanes);
+      pl.block = round_up64(lanes);
+      pl.tpb = pl.block;
+      pl.grid_x = std::min(ceil_div(workers(M, k), pl.rows_per_block), std::max<int64_t>(1, ceil_div(M, pl.rows_per_block)));
+      pl.grid_y = ceil_div(N, pl.slice);
+      pl.zero_fill = false;
+      pl.policy = SEGSCAN_WRITE_OWNER;
+      pl.ws_bytes = 0;
+      break;
+    }
+  }
+  if (pl.grid_y > 65535) return fail(SEGSCAN_EUNSUPPORTED, "N = %lld needs %lld column slices (> 65535)", (long long)N,
+                                     (long long)pl.grid_y);
+  if (pl.grid_x > kMaxI32) return fail(SEGSCAN_EUNSUPPORTED, "grid too large");
+  return SEGSCAN_OK;
+}
+
+segscan_status_t validate_csr(const segscan_csr_t* A, bool need_pointers) {
+  if (!A) return fail(SEGSCAN_EINVAL, "A is NULL");
+  if (A->nrow < 0 || A->ncol < 0 || A->nnz < 0) return fail(SEGSCAN_EINVAL, "negative matrix dimension");
+  if (A->nrow >= kMaxI32) return fail(SEGSCAN_EINVAL, "nrow %lld exceeds int32 indexing", (long long)A->nrow);
+  if (A->ncol > kMaxI32) return fail(SEGSCAN_EINVAL, "ncol %lld exceeds int32 indexing", (long long)A->ncol);
+  if (A->nnz > kMaxI32) return fail(SEGSCAN_EINVAL, "nnz %lld exceeds int32 indexing", (long long)A->nnz);
+  if (A->nrow == 0 && A->nnz > 0) return fail(SEGSCAN_EINVAL, "nnz > 0 with no rows");
+  if (A->ncol == 0 && A->nnz > 0) return fail(SEGSCAN_EINVAL, "nnz > 0 with no columns");
+  if (need_pointers) {
+    if (A->nrow > 0 && !A->rowptr) return fail(SEGSCAN_EINVAL, "rowptr is NULL");
+    if (A->nnz > 0 && !A->colind) return fail(SEGSCAN_EINVAL, "colind is NULL");
+  }
+  return SEGSCAN_OK;
+}
+
+void fill_info(const SpmmPlan& pl, segscan_launch_info_t* info) {
+  if (!info) return;
+  std::memset(info, 0, sizeof(*info));
+  info->variant = pl.variant;
+  info->group_size = pl.G;
+  info->tile_size = pl.T;
+  info->thread_per_block = pl.block;
+  info->coarsen_factor = pl.CF;
+  info->thread_nz = pl.tnz;
+  info->write_policy = pl.policy;
+  info->chunk = static_cast<int32_t>(pl.fam == kEbpr ? pl.T : pl.chunk);
+  info->grid_x = pl.grid_x;
+  info->grid_y = pl.grid_y;
+  info->nchunks = pl.fam == kEbpr ? pl.ntiles : pl.nchunks;
+  info->workspace_bytes = pl.ws_bytes;
+}
+
+VType vtype(segscan_dtype_t t) { return static_cast<VType>(static_cast<int>(t)); }
+
+// ---------------------------------------------------------------------------------------
+// segment_coo planning
+
+struct SegPlan {
+  int variant = 0;
+  bool pr = true;
+  int G = 1, T = 1, CF = 1, tnz = 1, policy = SEGSCAN_WRITE_ATOMIC;
+  int X = 1, P = 1, Y = 1, slice = 1, block = 64, tpb = 64;
+  int64_t grid_x = 1, grid_y = 1, ntiles = 0, chunk = 0, nchunks = 0, total_groups = 0;
+  bool zero_fill = true;
+  size_t ws_bytes = 0, off_crow = 0, off_cval = 0;
+};
+
+segscan_status_t plan_seg(int64_t E, int64_t N, segscan_dtype_t st, int variant, const Knobs& user, int cf_cap,
+                          SegPlan& pl) {
+  if (variant < 0 || variant > SEGSCAN_SEG_AUTO) return fail(SEGSCAN_EINVAL, "unknown segment_coo variant %d", variant);
+  const int vec = vec_elems(st);
+  Knobs k = user;
+  int v = variant;
+  if (v == SEGSCAN_SEG_AUTO) {
+    // Streaming src: a group of lanes per src row at 128-bit width, long owner chunks.
+    v = SEGSCAN_SEG_LESSATOM_SR;
+    int cf = std::min(vec, divisor_cf(N, vec));
+    int lanes = static_cast<int>(std::min<int64_t>(64, ceil_div(N, cf)));
+    int g = 1;
+    while (g < lanes) g <<= 1;
+    int gf = 0;
+    while ((1 << gf) < g) ++gf;
+    k = mk(gf == 0 ? -1 : gf, 5, 256, 1, 1, cf, std::max(1, 512 / g), SEGSCAN_WRITE_OWNER);
+    k.bd = std::max<int>(1, static_cast<int>(E / 2048));
+  }
+  pl.variant = v;
+  switch (v) {
+    case SEGSCAN_SEG_PR:  // main.cu:49 segment_coo<float,int,5,256,5,1,16>
+      defaults(k, mk(5, 5, 256, 1, 16, 0, 1, 0));
+      break;
+    case SEGSCAN_SEG_PR_NEW:  // main.cu:50 segment_coo_new<float,int,5,256,5,2>
+      defaults(k, mk(5, 5, 256, 1, 1, 0, 1, 0));
+      if (!k.block_dim_y) k.block_dim_y = 2;
+      break;
+    case SEGSCAN_SEG_SR:  // main.cu:51 segment_coo_sr<float,int,4,256,4,1,1,16>
+      defaults(k, mk(4, 5, 256, 1, 16, 4, 1, SEGSCAN_WRITE_ATOMIC));
+      break;
+    case SEGSCAN_SEG_NOSHMEM_SR:  // main.cu:52 segment_coo_noshmem_sr<float,int,4,256,4,2,1,16>
+      defaults(k, mk(4, 5, 256, 1, 16, 4, 2, SEGSCAN_WRITE_ATOMIC));
+      break;
+    case SEGSCAN_SEG_LESSATOM_SR:  // main.cu:53 segment_coo_noshmem_lessatom_sr<float,int,4,256,4,1,1,16>
+      defaults(k, mk(4, 5, 256, 1, 16, 4, 1, SEGSCAN_WRITE_LESSATOM));
+      break;
+    case SEGSCAN_SEG_SORTED_SR:  // main.cu:54 segment_coo_sorted_sr<float,int,2,4,8,2,16>
+      defaults(k, mk(0, 5, 256, 1, 1, 2, 4, SEGSCAN_WRITE_LESSATOM));
+      if (!k.dtile_size) k.dtile_size = 8;
+      if (!k.dtile_num) k.dtile_num = 2;
+      if (!k.block_dim_y) k.block_dim_y = 16;
+      break;
+    default:
+      return fail(SEGSCAN_EINVAL, "unknown segment_coo variant %d", variant);
+  }
+  segscan_status_t s = check_common_knobs(k);
+  if (s) return s;
+  const int gf = factor_value(k.group_factor), tf = factor_value(k.tile_factor);
+  if (gf > 6) return fail(SEGSCAN_EUNSUPPORTED, "group_factor %d: groups are at most 64 lanes (wave64)", gf);
+  if (tf > 10) return fail(SEGSCAN_EUNSUPPORTED, "tile_factor %d too large", tf);
+  if (k.cf && !(k.cf == 1 || k.cf == 2 || k.cf == 4 || k.cf == 8))
+    return fail(SEGSCAN_EUNSUPPORTED, "coarsen_factor %d not in {1,2,4,8}", k.cf);
+  if (k.policy < 0 || k.policy > SEGSCAN_WRITE_LESSATOM) return fail(SEGSCAN_EINVAL, "unknown write policy %d", k.policy);
+  pl.G = 1 << gf;
+  pl.T = 1 << tf;
+  pl.tnz = k.thread_nz;
+
+  if (v == SEGSCAN_SEG_PR || v == SEGSCAN_SEG_PR_NEW) {
+    pl.pr = true;
+    int cf = k.cf ? k.cf : divisor_cf(N, vec);
+    if (cf == 8 && st == SEGSCAN_F32) return fail(SEGSCAN_EUNSUPPORTED, "coarsen_factor 8 needs a 16-bit src");
+    cf = std::min(cf, cf_cap);
+    pl.CF = cf;
+    if (v == SEGSCAN_SEG_PR) {
+      if (pl.T > 64) return fail(SEGSCAN_EUNSUPPORTED, "tile_size %d: tiles are at most one wave (64)", pl.T);
+      if (pl.G > pl.T) return fail(SEGSCAN_EUNSUPPORTED, "group_size %d exceeds tile_size %d", pl.G, pl.T);
+      pl.X = pl.T;
+      pl.slice = std::max(pl.T, cf);
+      pl.P = static_cast<int>(ceil_div(std::min<int64_t>(N, pl.slice), cf));
+      if (pl.P * pl.X > 1024) {
+        pl.P = 1024 / pl.X;
+        pl.slice = pl.P * cf;
+      }
+      pl.Y = static_cast<int>(ceil_div(std::max(1, k.tpb / pl.T), pl.P));
+      pl.ntiles = ceil_div(E, pl.X);
+      pl.grid_x = std::min(ceil_div(workers(E, k), pl.Y), std::max<int64_t>(1, ceil_div(pl.ntiles, pl.Y)));
+    } else {
+      // segscan.cuh:327-355: x = elements, y = column panels of CF
+      const int max_bdy = std::max(1, k.tpb / pl.T);
+      const int64_t ntile_y = ceil_div(N, cf);
+      const int bdy = static_cast<int>(std::min<int64_t>(ntile_y, std::min(k.block_dim_y, max_bdy)));
+      int bdx = static_cast<int>(ceil_div(k.tpb, bdy));
+      if (bdx % pl.G) bdx = static_cast<int>(ceil_div(bdx, pl.G) * pl.G);  // whole groups
+      pl.X = bdx;
+      pl.P = bdy;
+      pl.Y = 1;
+      pl.slice = bdy * cf;
+      pl.ntiles = ceil_div(E, pl.X);
+      pl.grid_x = std::max<int64_t>(1, pl.ntiles);
+    }
+    const int lanes = pl.P * pl.X * pl.Y;
+    if (lanes > 1024) return fail(SEGSCAN_EUNSUPPORTED, "workgroup of %d lanes exceeds 1024", lanes);
+    pl.block = round_up64(lanes);
+    pl.tpb = pl.block;
+    pl.grid_y = ceil_div(N, pl.slice);
+    pl.policy = SEGSCAN_WRITE_ATOMIC;
+    pl.zero_fill = true;
+    pl.ws_bytes = 0;
+  } else {
+    pl.pr = false;
+    int cf, tpb;
+    if (v == SEGSCAN_SEG_SORTED_SR) {
+      // segscan.cuh:974-1008: x lanes = min(DtileNum, CEIL(N, Dtile*CF)) * Dtile over
+      // columns, y = BlockDimY element chunks of ThreadNz.
+      if (!is_pow2(k.dtile_size) || k.dtile_num < 1)
+        return fail(SEGSCAN_EUNSUPPORTED, "dtile_size %d must be a power of two, dtile_num >= 1", k.dtile_size);
+      cf = std::min(k.cf ? k.cf : 2, 4);
+      int c2 = 1;
+      while (c2 * 2 <= cf) c2 <<= 1;
+      cf = c2;
+      const int bdx = static_cast<int>(std::min<int64_t>(k.dtile_num, ceil_div(N, static_cast<int64_t>(k.dtile_size) * cf))) *
+                      k.dtile_size;
+      int g = 1;
+      while (g < bdx) g <<= 1;
+      if (g > 64) return fail(SEGSCAN_EUNSUPPORTED, "dtile_size * dtile_num = %d lanes exceeds a wave", bdx);
+      pl.G = g;
+      const int bdy = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(k.block_dim_y, ceil_div(E, pl.tnz))));
+      tpb = g * bdy;
+      if (tpb > 1024) return fail(SEGSCAN_EUNSUPPORTED, "workgroup of %d lanes exceeds 1024", tpb);
+      pl.chunk = pl.tnz;
+    } else {
+      // segscan.cuh:817-866: coarsen = min(CEIL(N, group), CoarsenFactor)
+      cf = static_cast<int>(std::min<int64_t>(ceil_div(N, pl.G), k.cf ? k.cf : 4));
+      int c2 = 1;
+      while (c2 * 2 <= cf) c2 <<= 1;
+      cf = c2;
+      tpb = std::max(k.tpb, pl.G);
+      pl.chunk = static_cast<int64_t>(pl.tnz) * pl.G;
+    }
+    pl.CF = std::min(cf, cf_cap);
+    pl.tpb = round_up64(tpb);
+    pl.block = pl.tpb;
+    pl.nchunks = ceil_div(E, pl.chunk);
+    const int gpb = pl.tpb / pl.G;
+    const int64_t want = (v == SEGSCAN_SEG_SORTED_SR) ? ceil_div(pl.nchunks, gpb) : workers(E, k);
+    pl.grid_x = std::min(want, std::max<int64_t>(1, ceil_div(pl.nchunks, gpb)));
+    pl.total_groups = pl.grid_x * gpb;
+    pl.grid_y = ceil_div(N, static_cast<int64_t>(pl.G) * pl.CF);
+    pl.policy = k.policy ? k.policy : SEGSCAN_WRITE_ATOMIC;
+    pl.zero_fill = pl.policy != SEGSCAN_WRITE_OWNER;
+    size_t off = 0;
+    if (pl.policy == SEGSCAN_WRITE_OWNER) {
+      pl.off_crow = off;
+      off += align_up(static_cast<size_t>(std::max<int64_t>(pl.nchunks, 1)) * 4);
+      pl.off_cval = off;
+      off += align_up(static_cast<size_t>(std::max<int64_t>(pl.nchunks, 1)) * static_cast<size_t>(N) * 4);
+    }
+    pl.ws_bytes = off;
+  }
+  if (pl.grid_y > 65535) return fail(SEGSCAN_EUNSUPPORTED, "N = %lld needs too many column slices", (long long)N);
+  if (pl.grid_x > kMaxI32) return fail(SEGSCAN_EUNSUPPORTED, "grid too large");
+  return SEGSCAN_OK;
+}
+
+void fill_info(const SegPlan& pl, segscan_launch_info_t* info) {
+  if (!info) return;
+  std::memset(info, 0, sizeof(*info));
+  info->variant = pl.variant;
+  info->group_size = pl.G;
+  info->tile_size = pl.pr ? pl.X : pl.T;
+  info->thread_per_block = pl.block;
+  info->coarsen_factor = pl.CF;
+  info->thread_nz = pl.tnz;
+  info->write_policy = pl.policy;
+  info->chunk = static_cast<int32_t>(pl.pr ? pl.X : pl.chunk);
+  info->grid_x = pl.grid_x;
+  info->grid_y = pl.grid_y;
+  info->nchunks = pl.pr ? pl.ntiles : pl.nchunks;
+  info->workspace_bytes = pl.ws_bytes;
+}
+
+}  // namespace
+
+// =========================================================================================
+// C ABI
+
+extern "C" {
+
+int32_t segscan_abi_version(void) { return 1; }
+
+const char* segscan_status_string(segscan_status_t s) {
+  switch (s) {
+    case SEGSCAN_OK: return "SEGSCAN_OK";
+    case SEGSCAN_EINVAL: return "SEGSCAN_EINVAL: invalid argument";
+    case SEGSCAN_EUNSUPPORTED: return "SEGSCAN_EUNSUPPORTED: unsupported knob combination";
+    case SEGSCAN_EHIP: return "SEGSCAN_EHIP: HIP runtime error";
+    case SEGSCAN_EWORKSPACE: return "SEGSCAN_EWORKSPACE: workspace missing or too small";
+  }
+  return "unknown segscan status";
+}
+
+const char* segscan_last_error(void) { return g_last_error.c_str(); }
+
+segscan_status_t segscan_spmm_describe(const segscan_csr_t* A, int64_t N, segscan_dtype_t b_dtype,
+                                       segscan_spmm_variant_t variant, const segscan_knobs_t* knobs,
+                                       segscan_launch_info_t* info) {
+  segscan_status_t st = validate_csr(A, false);
+  if (st) return st;
+  if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N = %lld out of range", (long long)N);
+  if (!dtype_valid(b_dtype)) return fail(SEGSCAN_EINVAL, "unknown dtype %d", (int)b_dtype);
+  SpmmPlan pl;
+  st = plan_spmm(A->nrow, A->nnz, std::max<int64_t>(N, 1), b_dtype, variant, from_c(knobs), 8, pl);
+  if (st) return st;
+  fill_info(pl, info);
+  return ok();
+}
+
+segscan_status_t segscan_spmm_workspace_size(const segscan_csr_t* A, int64_t N, segscan_dtype_t b_dtype,
+                                             segscan_spmm_variant_t variant, const segscan_knobs_t* knobs,
+                                             size_t* bytes) {
+  if (!bytes) return fail(SEGSCAN_EINVAL, "bytes is NULL");
+  segscan_launch_info_t info;
+  segscan_status_t st = segscan_spmm_describe(A, N, b_dtype, variant, knobs, &info);
+  if (st) return st;
+  *bytes = info.workspace_bytes;
+  return ok();
+}
+
+segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N, const void* B, int64_t ldb,
+                                  segscan_dtype_t b_dtype, float* C, int64_t ldc, segscan_spmm_variant_t variant,
+                                  const segscan_knobs_t* knobs, uint32_t flags, void* workspace,
+                                  size_t workspace_bytes, void* stream) {
+  segscan_status_t st = validate_csr(A, true);
+  if (st) return st;
+  if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N = %lld out of range", (long long)N);
+  if (!dtype_valid(b_dtype)) return fail(SEGSCAN_EINVAL, "unknown dtype %d", (int)b_dtype);
+  if (flags & ~SEGSCAN_ACCUMULATE) return fail(SEGSCAN_EINVAL, "unknown flags 0x%x", flags);
+  const int64_t M = A->nrow, nnz = A->nnz;
+  if (M > 0 && N > 0) {
+    if (!C) return fail(SEGSCAN_EINVAL, "C is NULL");
+    if (ldc < N) return fail(SEGSCAN_EINVAL, "ldc %lld < N %lld", (long long)ldc, (long long)N);
+    if (reinterpret_cast<uintptr_t>(C) % 4) return fail(SEGSCAN_EINVAL, "C is not 4-byte aligned");
+  }
+  if (nnz > 0 && N > 0) {
+    if (!B) return fail(SEGSCAN_EINVAL, "B is NULL");
+    if (ldb < N) return fail(SEGSCAN_EINVAL, "ldb %lld < N %lld", (long long)ldb, (long long)N);
+    if (reinterpret_cast<uintptr_t>(B) % dtype_size(b_dtype)) return fail(SEGSCAN_EINVAL, "B is misaligned");
+  }
+  const bool accumulate = flags & SEGSCAN_ACCUMULATE;
+  const int cap = aligned_cf(8, B, std::max<int64_t>(ldb, 1), dtype_size(b_dtype), C, std::max<int64_t>(ldc, 1));
+  SpmmPlan pl;
+  st = plan_spmm(M, nnz, std::max<int64_t>(N, 1), b_dtype, variant, from_c(knobs), cap, pl);
+  if (st) return st;
+  if (M == 0 || N == 0) return ok();
+  if (pl.ws_bytes > 0 && nnz > 0) {
+    if (!workspace) return fail(SEGSCAN_EWORKSPACE, "workspace of %zu bytes required", pl.ws_bytes);
+    if (workspace_bytes < pl.ws_bytes)
+      return fail(SEGSCAN_EWORKSPACE, "workspace %zu bytes < %zu required", workspace_bytes, pl.ws_bytes);
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (nnz == 0) {
+    if (!accumulate) return hip_status(launch_fill_zero(C, M, N, ldc, s), "zero fill");
+    return ok();
+  }
+  if (pl.zero_fill && !accumulate) {
+    hipError_t e = launch_fill_zero(C, M, N, ldc, s);
+    if (e != hipSuccess) return hip_status(e, "zero fill");
+  }
+  char* ws = static_cast<char*>(workspace);
+  const VType vt = vtype(b_dtype);
+  bool compiled = true;
+  hipError_t e = hipSuccess;
+  Launch l;
+  l.grid = dim3(static_cast<unsigned>(pl.grid_x), static_cast<unsigned>(pl.grid_y));
+  l.block = dim3(pl.block);
+  l.shmem = 0;
+  l.stream = s;
+  switch (pl.fam) {
+    case kEbpr:
+    case kRbpr: {
+      PrParams p{};
+      p.M = static_cast<int>(M);
+      p.N = static_cast<int>(N);
+      p.nnz = nnz;
+      p.rowptr = A->rowptr;
+      p.colind = A->colind;
+      p.vals = A->vals;
+      p.B = B;
+      p.ldb = ldb;
+      p.C = C;
+      p.ldc = ldc;
+      p.X = pl.X;
+      p.P = pl.P;
+      p.Y = pl.Y;
+      p.G = pl.G;
+      p.slice = pl.slice;
+      p.ntiles = pl.ntiles;
+      p.row_store = pl.policy == SEGSCAN_WRITE_OWNER;
+      p.accumulate = accumulate;
+      if (pl.fam == kEbpr) {
+        p.tile_row = reinterpret_cast<int32_t*>(ws + pl.off_rows);
+        e = launch_partition_rows(A->rowptr, p.M, nnz, pl.T, pl.ntiles, const_cast<int32_t*>(p.tile_row), nullptr, s);
+        if (e != hipSuccess) return hip_status(e, "partition");
+      }
+      e = launch_spmm_pr(vt, pl.CF, pl.G, p, l, pl.fam == kEbpr, compiled);
+      break;
+    }
+    case kEbsr: {
+      SrParams p{};
+      p.M = static_cast<int>(M);
+      p.N = static_cast<int>(N);
+      p.nnz = nnz;
+      p.rowptr = A->rowptr;
+      p.colind = A->colind;
+      p.vals = A->vals;
+      p.B = B;
+      p.ldb = ldb;
+      p.C = C;
+      p.ldc = ldc;
+      p.tpb = pl.tpb;
+      p.chunk = pl.chunk;
+      p.nchunks = pl.nchunks;
+      p.total_groups = pl.total_groups;
+      p.chunk_row = reinterpret_cast<int32_t*>(ws + pl.off_rows);
+      p.chunk_prev = reinterpret_cast<int32_t*>(ws + pl.off_prev);
+      p.policy = pl.policy;
+      p.accumulate = accumulate;
+      if (pl.policy == SEGSCAN_WRITE_OWNER) {
+        p.carry_row = reinterpret_cast<int32_t*>(ws + pl.off_crow);
+        p.carry_val = reinterpret_cast<float*>(ws + pl.off_cval);
+      }
+      e = launch_partition_rows(A->rowptr, p.M, nnz, pl.chunk, pl.nchunks, const_cast<int32_t*>(p.chunk_row),
+                                const_cast<int32_t*>(p.chunk_prev), s);
+      if (e != hipSuccess) return hip_status(e, "partition");
+      l.shmem = static_cast<size_t>(pl.tpb) * 12;
+      e = launch_spmm_sr(vt, pl.CF, pl.G, p, l, compiled);
+      if (e == hipSuccess && compiled && pl.policy == SEGSCAN_WRITE_OWNER)
+        e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, C, ldc, s);
+      break;
+    }
+    case kRbsr: {
+      RbsrParams p{};
+      p.M = static_cast<int>(M);
+      p.N = static_cast<int>(N);
+      p.rowptr = A->rowptr;
+      p.colind = A->colind;
+      p.vals = A->vals;
+      p.B = B;
+      p.ldb = ldb;
+      p.C = C;
+      p.ldc = ldc;
+      p.tpr = pl.tpr;
+      p.rows_per_block = pl.rows_per_block;
+      p.slice = pl.slice;
+      p.rows_total = M;
+      p.accumulate = accumulate;
+      e = launch_spmm_rbsr(vt, pl.CF, p, l, compiled);
+      break;
+    }
+  }
+  if (!compiled)
+    return fail(SEGSCAN_EUNSUPPORTED, "no compiled kernel for dtype %d, coarsen %d, group %d", (int)b_dtype, pl.CF, pl.G);
+  return hip_status(e, "SpMM launch");
+}
+
+segscan_status_t segscan_segment_coo_describe(int64_t E, int64_t N, int64_t dst_len, segscan_dtype_t src_dtype,
+                                              segscan_seg_variant_t variant, const segscan_knobs_t* knobs,
+                                              segscan_launch_info_t* info) {
+  if (E < 0 || E > kMaxI32) return fail(SEGSCAN_EINVAL, "E = %lld out of range", (long long)E);
+  if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N = %lld out of range", (long long)N);
+  if (dst_len < 0 || dst_len >= kMaxI32) return fail(SEGSCAN_EINVAL, "dst_len = %lld out of range", (long long)dst_len);
+  if (!dtype_valid(src_dtype)) return fail(SEGSCAN_EINVAL, "unknown dtype %d", (int)src_dtype);
+  SegPlan pl;
+  segscan_status_t st = plan_seg(E, std::max<int64_t>(N, 1), src_dtype, variant, from_c(knobs), 8, pl);
+  if (st) return st;
+  fill_info(pl, info);
+  return ok();
+}
+
+segscan_status_t segscan_segment_coo_workspace_size(int64_t E, int64_t N, int64_t dst_len, segscan_dtype_t src_dtype,
+                                                    segscan_seg_variant_t variant, const segscan_knobs_t* knobs,
+                                                    size_t* bytes) {
+  if (!bytes) return fail(SEGSCAN_EINVAL, "bytes is NULL");
+  segscan_launch_info_t info;
+  segscan_status_t st = segscan_segment_coo_describe(E, N, dst_len, src_dtype, variant, knobs, &info);
+  if (st) return st;
+  *bytes = info.workspace_bytes;
+  return ok();
+}
+
+segscan_status_t segscan_segment_coo(const void* src, int64_t ld_src, segscan_dtype_t src_dtype, const int32_t* index,
+                                     int64_t E, int64_t N, float* dst, int64_t ld_dst, int64_t dst_len,
+                                     segscan_seg_variant_t variant, const segscan_knobs_t* knobs, uint32_t flags,
+                                     void* workspace, size_t workspace_bytes, void* stream) {
+  if (E < 0 || E > kMaxI32) return fail(SEGSCAN_EINVAL, "E = %lld out of range", (long long)E);
+  if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N = %lld out of range", (long long)N);
+  if (dst_len < 0 || dst_len >= kMaxI32) return fail(SEGSCAN_EINVAL, "dst_len = %lld out of range", (long long)dst_len);
+  if (!dtype_valid(src_dtype)) return fail(SEGSCAN_EINVAL, "unknown dtype %d", (int)src_dtype);
+  if (flags & ~SEGSCAN_ACCUMULATE) return fail(SEGSCAN_EINVAL, "unknown flags 0x%x", flags);
+  if (E > 0 && dst_len == 0) return fail(SEGSCAN_EINVAL, "elements present but dst_len is 0");
+  if (E > 0 && N > 0) {
+    if (!src) return fail(SEGSCAN_EINVAL, "src is NULL");
+    if (!index) return fail(SEGSCAN_EINVAL, "index is NULL");
+    if (ld_src < N) return fail(SEGSCAN_EINVAL, "ld_src %lld < N %lld", (long long)ld_src, (long long)N);
+    if (reinterpret_cast<uintptr_t>(src) % dtype_size(src_dtype)) return fail(SEGSCAN_EINVAL, "src is misaligned");
+  }
+  if (dst_len > 0 && N > 0) {
+    if (!dst) return fail(SEGSCAN_EINVAL, "dst is NULL");
+    if (ld_dst < N) return fail(SEGSCAN_EINVAL, "ld_dst %lld < N %lld", (long long)ld_dst, (long long)N);
+    if (reinterpret_cast<uintptr_t>(dst) % 4) return fail(SEGSCAN_EINVAL, "dst is not 4-byte aligned");
+  }
+  const bool accumulate = flags & SEGSCAN_ACCUMULATE;
+  const int cap = aligned_cf(8, src, std::max<int64_t>(ld_src, 1), dtype_size(src_dtype), dst, std::max<int64_t>(ld_dst, 1));
+  SegPlan pl;
+  segscan_status_t st = plan_seg(E, std::max<int64_t>(N, 1), src_dtype, variant, from_c(knobs), cap, pl);
+  if (st) return st;
+  if (dst_len == 0 || N == 0) return ok();
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (E == 0) {
+    if (!accumulate) return hip_status(launch_fill_zero(dst, dst_len, N, ld_dst, s), "zero fill");
+    return ok();
+  }
+  if (pl.ws_bytes > 0) {
+    if (!workspace) return fail(SEGSCAN_EWORKSPACE, "workspace of %zu bytes required", pl.ws_bytes);
+    if (workspace_bytes < pl.ws_bytes)
+      return fail(SEGSCAN_EWORKSPACE, "workspace %zu bytes < %zu required", workspace_bytes, pl.ws_bytes);
+  }
+  if (pl.zero_fill && !accumulate) {
+    hipError_t e = launch_fill_zero(dst, dst_len, N, ld_dst, s);
+    if (e != hipSuccess) return hip_status(e, "zero fill");
+  }
+  const VType vt = vtype(src_dtype);
+  bool compiled = true;
+  hipError_t e;
+  Launch l;
+  l.grid = dim3(static_cast<unsigned>(pl.grid_x), static_cast<unsigned>(pl.grid_y));
+  l.block = dim3(pl.block);
+  l.shmem = 0;
+  l.stream = s;
+  char* ws = static_cast<char*>(workspace);
+  if (pl.pr) {
+    PrParams p{};
+    p.M = static_cast<int>(dst_len);
+    p.N = static_cast<int>(N);
+    p.nnz = E;
+    p.index = index;
+    p.B = src;
+    p.ldb = ld_src;
+    p.C = dst;
+    p.ldc = ld_dst;
+    p.X = pl.X;
+    p.P = pl.P;
+    p.Y = pl.Y;
+    p.G = pl.G;
+    p.slice = pl.slice;
+    p.ntiles = pl.ntiles;
+    p.accumulate = accumulate;
+    e = launch_seg_pr(vt, pl.CF, pl.G, p, l, compiled);
+  } else {
+    SrParams p{};
+    p.M = static_cast<int>(dst_len);
+    p.N = static_cast<int>(N);
+    p.nnz = E;
+    p.index = index;
+    p.B = src;
+    p.ldb = ld_src;
+    p.C = dst;
+    p.ldc = ld_dst;
+    p.tpb = pl.tpb;
+    p.chunk = pl.chunk;
+    p.nchunks = pl.nchunks;
+    p.total_groups = pl.total_groups;
+    p.policy = pl.policy;
+    p.accumulate = accumulate;
+    if (pl.policy == SEGSCAN_WRITE_OWNER) {
+      p.carry_row = reinterpret_cast<int32_t*>(ws + pl.off_crow);
+      p.carry_val = reinterpret_cast<float*>(ws + pl.off_cval);
+    }
+    l.shmem = static_cast<size_t>(pl.tpb) * 12;
+    e = launch_seg_sr(vt, pl.CF, pl.G, p, l, compiled);
+    if (e == hipSuccess && compiled && pl.policy == SEGSCAN_WRITE_OWNER)
+      e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, dst, ld_dst, s);
+  }
+  if (!compiled)
+    return fail(SEGSCAN_EUNSUPPORTED, "no compiled kernel for dtype %d, coarsen %d, group %d", (int)src_dtype, pl.CF, pl.G);
+  return hip_status(e, "segment_coo launch");
+}
+
+// ---- backward pieces -------------------------------------------------------------------
+
+segscan_status_t segscan_csr_transpose_workspace_size(const segscan_csr_t* A, size_t* bytes) {
+  if (!bytes) return fail(SEGSCAN_EINVAL, "bytes is NULL");
+  segscan_status_t st = validate_csr(A, false);
+  if (st) return st;
+  const int64_t K = A->ncol;
+  const int64_t nblocks = ceil_div(std::max<int64_t>(K, 1), 1024);
+  *bytes = align_up(static_cast<size_t>(std::max<int64_t>(K, 1)) * 4) + align_up(static_cast<size_t>(nblocks) * 4);
+  return ok();
+}
+
+segscan_status_t segscan_csr_transpose(const segscan_csr_t* A, int32_t* colptr, int32_t* rowind, int32_t* perm,
+                                       float* vals_t, void* workspace, size_t workspace_bytes, void* stream) {
+  segscan_status_t st = validate_csr(A, true);
+  if (st) return st;
+  if (!colptr) return fail(SEGSCAN_EINVAL, "colptr is NULL");
+  if (A->nnz > 0 && (!rowind || !perm)) return fail(SEGSCAN_EINVAL, "rowind / perm is NULL");
+  size_t need = 0;
+  segscan_csr_transpose_workspace_size(A, &need);
+  if (!workspace) return fail(SEGSCAN_EWORKSPACE, "workspace of %zu bytes required", need);
+  if (workspace_bytes < need) return fail(SEGSCAN_EWORKSPACE, "workspace %zu bytes < %zu required", workspace_bytes, need);
+  const int64_t K = A->ncol;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t nblocks = ceil_div(K, 1024);
+  char* ws = static_cast<char*>(workspace);
+  int32_t* counts = reinterpret_cast<int32_t*>(ws);
+  int32_t* bsums = reinterpret_cast<int32_t*>(ws + align_up(static_cast<size_t>(K) * 4));
+  hipError_t e = launch_transpose(A->rowptr, A->colind, A->vals, static_cast<int>(A->nrow), static_cast<int>(K), A->nnz,
+                                  colptr, rowind, perm, vals_t, counts, bsums, nblocks, s);
+  return hip_status(e, "transpose");
+}
+
+segscan_status_t segscan_sddmm_csr(const segscan_csr_t* A, int64_t N, const void* X, int64_t ld_x, const void* Y,
+                                   int64_t ld_y, segscan_dtype_t xy_dtype, float* out, void* stream) {
+  segscan_status_t st = validate_csr(A, true);
+  if (st) return st;
+  if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N out of range");
+  if (!dtype_valid(xy_dtype)) return fail(SEGSCAN_EINVAL, "unknown dtype %d", (int)xy_dtype);
+  if (A->nnz == 0) return ok();
+  if (!out) return fail(SEGSCAN_EINVAL, "out is NULL");
+  if (N > 0) {
+    if (!X || !Y) return fail(SEGSCAN_EINVAL, "X / Y is NULL");
+    if (ld_x < N || ld_y < N) return fail(SEGSCAN_EINVAL, "leading dimension < N");
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (N == 0) {
+    return hip_status(launch_fill_zero(out, 1, A->nnz, A->nnz, s), "sddmm zero");
+  }
+  hipError_t e = launch_sddmm(vtype(xy_dtype), A->rowptr, A->colind, static_cast<int>(A->nrow), A->nnz,
+                              static_cast<int>(N), X, ld_x, Y, ld_y, out, s);
+  return hip_status(e, "sddmm");
+}
+
+segscan_status_t segscan_segment_gather(const float* ddst, int64_t ld_ddst, const int32_t* index, int64_t E, int64_t N,
+                                        const int32_t* counts, float* dsrc, int64_t ld_dsrc, void* stream) {
+  if (E < 0 || E > kMaxI32 || N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "size out of range");
+  if (E == 0 || N == 0) return ok();
+  if (!ddst || !index || !dsrc) return fail(SEGSCAN_EINVAL, "NULL pointer");
+  if (ld_ddst < N || ld_dsrc < N) return fail(SEGSCAN_EINVAL, "leading dimension < N");
+  return hip_status(launch_segment_gather(ddst, ld_ddst, index, E, static_cast<int>(N), counts, dsrc, ld_dsrc,
+                                          static_cast<hipStream_t>(stream)),
+                    "segment gather");
+}
+
+segscan_status_t segscan_segment_counts(const int32_t* index, int64_t E, int64_t dst_len, int32_t* counts,
+                                        void* stream) {
+  if (E < 0 || E > kMaxI32 || dst_len < 0 || dst_len >= kMaxI32) return fail(SEGSCAN_EINVAL, "size out of range");
+  if (dst_len == 0) return ok();
+  if (!counts || (E > 0 && !index)) return fail(SEGSCAN_EINVAL, "NULL pointer");
+  return hip_status(launch_segment_counts(index, E, dst_len, counts, static_cast<hipStream_t>(stream)), "counts");
+}
+
+segscan_status_t segscan_check_csr(const segscan_csr_t* A, int32_t* flags_dev, void* stream) {
+  segscan_status_t st = validate_csr(A, true);
+  if (st) return st;
+  if (!flags_dev) return fail(SEGSCAN_EINVAL, "flags_dev is NULL");
+  return hip_status(launch_check_csr(A->rowptr, A->colind, A->nrow, A->ncol, A->nnz, flags_dev,
+                                     static_cast<hipStream_t>(stream)),
+                    "check_csr");
+}
+
+segscan_status_t segscan_check_index(const int32_t* index, int64_t E, int64_t dst_len, int32_t* flags_dev,
+                                     void* stream) {
+  if (E < 0 || E > kMaxI32 || dst_len < 0) return fail(SEGSCAN_EINVAL, "size out of range");
+  if (!flags_dev || (E > 0 && !index)) return fail(SEGSCAN_EINVAL, "NULL pointer");
+  return hip_status(launch_check_index(index, E, dst_len, flags_dev, static_cast<hipStream_t>(stream)), "check_index");
+}
+
+}  // extern "C"

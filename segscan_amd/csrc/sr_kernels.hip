@@ -1,0 +1,230 @@
+// Sequential-reduction kernels: element-balanced SpMM (ebsr, the reference's
+// "row caching" kernels test-cg/include/spmm/spmm.cuh:345-591 and
+// spmm_cg.cuh:376-591) and the segment_coo_sr family
+// (new-cg/include/segscan/segscan.cuh:357-815).
+//
+// A group of G lanes covers G*CF contiguous columns (one 128-bit load per lane
+// at CF=4 fp32 or CF=8 bf16/f16, so a 16-lane group reads a 256-byte dense row
+// in one instruction) and walks a chunk of consecutive nnz in order,
+// accumulating in registers and flushing when the row changes. Per tile of G
+// nnz every lane stages one (column, value, row) triple in LDS; the group then
+// reads them back as broadcast 128-bit LDS loads and issues up to 8 dense-row
+// loads before consuming any, to keep several HBM/MALL requests in flight.
+//
+// Write policies:
+//  * atomic: every flushed row is an atomic add (output zero-filled beforehand);
+//  * owner:  chunk c owns rows (row(c0-1), row(c1-1)] — including empty rows,
+//            which it zero-fills — and stores each once; a row that started in an
+//            earlier chunk is a carry, written to carry_val[c] and added by a
+//            fix-up pass. No atomics on the main path and no zero-fill pass;
+//  * lessatom: rows that start and end inside the chunk are plain stores, the
+//            chunk's first and last rows atomic adds (output zero-filled first).
+#include "device_common.hpp"
+#include "kernel_params.hpp"
+
+namespace segscan {
+
+template <typename VT, int CF, int G, bool SEG>
+__global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
+  extern __shared__ int32_t sr_smem[];
+  int32_t* s_k = sr_smem;              // [tpb] column (SpMM)
+  float* s_v = reinterpret_cast<float*>(sr_smem + p.tpb);  // [tpb] value (SpMM)
+  int32_t* s_r = sr_smem + 2 * p.tpb;  // [tpb] row / segment
+
+  constexpr int U = (G < 8) ? G : 8;   // dense-row loads in flight per group
+  const int L = threadIdx.x;
+  const int g = L / G;
+  const int g_lane = L - g * G;
+  const int groups_per_block = p.tpb / G;
+  const int64_t ggid = static_cast<int64_t>(blockIdx.x) * groups_per_block + g;
+  const int col0 = blockIdx.y * (G * CF) + g_lane * CF;
+  const int nv = min(CF, p.N - col0);
+  const VT* __restrict__ B = static_cast<const VT*>(p.B);
+  const int gbase = g * G;             // this group's LDS slots
+  const bool acc_mode = p.accumulate != 0;
+
+  for (int64_t c = ggid;; c += p.total_groups) {
+    // Chunks grow with the lane index; the wave's first lane has the lowest one.
+    if (__ballot(c < p.nchunks) == 0) break;
+    const bool chunk_valid = c < p.nchunks;
+    const int64_t c0 = c * p.chunk;
+    const int64_t clen = chunk_valid ? min(p.chunk, p.nnz - c0) : 0;
+
+    // Chunk entry state.
+    int first_row = 0, prev_row = -1;
+    if (chunk_valid) {
+      if constexpr (SEG) {
+        first_row = p.index[c0];
+        prev_row = (c0 > 0) ? p.index[c0 - 1] : -1;
+      } else {
+        first_row = p.chunk_row[c];
+        prev_row = p.chunk_prev[c];
+      }
+    }
+    const bool owner = p.policy == 2;
+    const bool is_carry = owner && chunk_valid && c0 > 0 && prev_row == first_row;
+    if (owner && chunk_valid) {
+      if (g_lane == 0 && blockIdx.y == 0) p.carry_row[c] = is_carry ? first_row : -1;
+      if (!acc_mode && nv > 0) {
+        for (int r = prev_row + 1; r < first_row; ++r) zero_cols<CF>(p.C + static_cast<int64_t>(r) * p.ldc + col0, nv);
+      }
+    }
+    int cur = first_row;
+    float acc[CF];
+#pragma unroll
+    for (int i = 0; i < CF; ++i) acc[i] = 0.0f;
+
+    // `last` marks the flush at the end of the chunk.
+    auto flush = [&](int row, bool last) {
+      if (nv <= 0) return;
+      float* dst = p.C + static_cast<int64_t>(row) * p.ldc + col0;
+      if (owner) {
+        if (row == first_row && is_carry) store_cols<CF>(p.carry_val + c * p.N + col0, nv, acc, false);
+        else store_cols<CF>(dst, nv, acc, acc_mode);
+      } else if (p.policy == 3 && !last && row != first_row) {
+        store_cols<CF>(dst, nv, acc, acc_mode);
+      } else {
+        atomic_add_cols<CF>(dst, nv, acc);
+      }
+    };
+
+    // Row hint for the wave-cooperative lookup: the wave's lowest nnz so far.
+    int wave_r0 = 0;
+    if constexpr (!SEG) wave_r0 = __builtin_amdgcn_readfirstlane(first_row);
+    const int64_t wave_c0 = static_cast<int64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(c))) * p.chunk;
+
+    const int ntile = static_cast<int>((p.chunk + G - 1) / G);  // wave-uniform
+    for (int j = 0; j < ntile; ++j) {
+      const int64_t off = static_cast<int64_t>(j) * G;
+      if (__ballot(off < clen) == 0) break;   // no group of this wave has entries left
+      const bool valid = off + g_lane < clen;
+      const int64_t e = c0 + off + g_lane;
+      int row;
+      if constexpr (SEG) {
+        row = valid ? p.index[e] : INT32_MAX;
+        s_r[L] = row;
+      } else {
+        const int64_t e_lo = min(wave_c0 + off, p.nnz - 1);
+        const int e_c = static_cast<int>(valid ? e : e_lo);
+        row = wave_row_lookup(p.rowptr, p.M, wave_r0, e_c);
+        wave_r0 = __builtin_amdgcn_readfirstlane(row);
+        s_r[L] = valid ? row : INT32_MAX;
+        s_k[L] = valid ? p.colind[e] : 0;
+        s_v[L] = valid ? (p.vals ? p.vals[e] : 1.0f) : 0.0f;
+      }
+      // The triples are read back only by lanes of the same wave: order the LDS
+      // writes before the reads without a workgroup barrier.
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+      const int cnt = static_cast<int>(max<int64_t>(0, min<int64_t>(G, clen - off)));
+      for (int pp = 0; pp < cnt; pp += U) {
+        int kk[U], rr[U];
+        float vv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          rr[u] = s_r[gbase + pp + u];
+          if constexpr (SEG) {
+            kk[u] = 0;
+            vv[u] = 1.0f;
+          } else {
+            kk[u] = s_k[gbase + pp + u];
+            vv[u] = s_v[gbase + pp + u];
+          }
+        }
+        float b[U][CF];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (pp + u < cnt && nv > 0) {
+            const int64_t krow = SEG ? (c0 + off + pp + u) : static_cast<int64_t>(kk[u]);
+            load_cols_any<VT, CF>(B + krow * p.ldb + col0, nv, b[u]);
+          } else {
+#pragma unroll
+            for (int i = 0; i < CF; ++i) b[u][i] = 0.0f;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (pp + u < cnt) {
+            const int r = rr[u];
+            if (r != cur) {
+              flush(cur, false);
+              if (owner && !acc_mode && nv > 0) {
+                for (int z = cur + 1; z < r; ++z) zero_cols<CF>(p.C + static_cast<int64_t>(z) * p.ldc + col0, nv);
+              }
+              cur = r;
+#pragma unroll
+              for (int i = 0; i < CF; ++i) acc[i] = 0.0f;
+            }
+#pragma unroll
+            for (int i = 0; i < CF; ++i) acc[i] += vv[u] * b[u][i];
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();  // LDS slots are rewritten by the next tile
+    }
+    if (chunk_valid && clen > 0) {
+      flush(cur, true);
+      // The last chunk owns the trailing empty rows.
+      if (owner && !acc_mode && nv > 0 && c == p.nchunks - 1) {
+        for (int z = cur + 1; z < p.M; ++z) zero_cols<CF>(p.C + static_cast<int64_t>(z) * p.ldc + col0, nv);
+      }
+    }
+  }
+}
+
+// ---- instantiation and dispatch ----------------------------------------------------------
+
+template <typename VT, int CF, int G>
+static hipError_t launch_sr_t(const SrParams& p, const Launch& l, bool seg) {
+  if (seg) sr_kernel<VT, CF, G, true><<<l.grid, l.block, l.shmem, l.stream>>>(p);
+  else sr_kernel<VT, CF, G, false><<<l.grid, l.block, l.shmem, l.stream>>>(p);
+  return hipGetLastError();
+}
+
+template <typename VT, int CF>
+static hipError_t launch_sr_g(int g, const SrParams& p, const Launch& l, bool seg, bool& ok) {
+  ok = true;
+  switch (g) {
+    case 1: return launch_sr_t<VT, CF, 1>(p, l, seg);
+    case 2: return launch_sr_t<VT, CF, 2>(p, l, seg);
+    case 4: return launch_sr_t<VT, CF, 4>(p, l, seg);
+    case 8: return launch_sr_t<VT, CF, 8>(p, l, seg);
+    case 16: return launch_sr_t<VT, CF, 16>(p, l, seg);
+    case 32: return launch_sr_t<VT, CF, 32>(p, l, seg);
+    case 64: return launch_sr_t<VT, CF, 64>(p, l, seg);
+    default: ok = false; return hipSuccess;
+  }
+}
+
+template <typename VT>
+static hipError_t launch_sr_cf(int cf, int g, const SrParams& p, const Launch& l, bool seg, bool& ok) {
+  switch (cf) {
+    case 1: return launch_sr_g<VT, 1>(g, p, l, seg, ok);
+    case 2: return launch_sr_g<VT, 2>(g, p, l, seg, ok);
+    case 4: return launch_sr_g<VT, 4>(g, p, l, seg, ok);
+    case 8: return launch_sr_g<VT, 8>(g, p, l, seg, ok);
+    default: ok = false; return hipSuccess;
+  }
+}
+
+static hipError_t launch_sr_vt(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool seg, bool& ok) {
+  switch (vt) {
+    case VType::F32: return launch_sr_cf<float>(cf, g, p, l, seg, ok);
+    case VType::BF16: return launch_sr_cf<bf16_t>(cf, g, p, l, seg, ok);
+    case VType::F16: return launch_sr_cf<f16_t>(cf, g, p, l, seg, ok);
+  }
+  ok = false;
+  return hipSuccess;
+}
+
+hipError_t launch_spmm_sr(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok) {
+  return launch_sr_vt(vt, cf, g, p, l, false, ok);
+}
+
+hipError_t launch_seg_sr(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok) {
+  return launch_sr_vt(vt, cf, g, p, l, true, ok);
+}
+
+}  // namespace segscan

@@ -1,0 +1,244 @@
+"""Torch-facing wrappers over the C ABI: CSR SpMM, segment_coo and the backward
+pieces. Device memory comes from torch; every call goes to libsegscan_hip.so on
+torch's current HIP stream. There is no CPU path: CPU tensors are rejected.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_size_t
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+
+from . import _abi
+
+_DTYPES = {torch.float32: _abi.F32, torch.bfloat16: _abi.BF16, torch.float16: _abi.F16}
+
+
+def _dtype_code(t: torch.Tensor) -> int:
+    try:
+        return _DTYPES[t.dtype]
+    except KeyError:
+        raise TypeError(f"dense operand dtype {t.dtype} not supported (float32, bfloat16, float16)") from None
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _require_device(*ts):
+    for t in ts:
+        if t is not None and not t.is_cuda:
+            raise ValueError("segscan_amd operates on GPU tensors only (no CPU fallback)")
+
+
+def _leading_dim(t: torch.Tensor, name: str) -> int:
+    if t.dim() != 2:
+        raise ValueError(f"{name} must be 2-D")
+    if t.stride(1) != 1 and t.size(1) > 1:
+        raise ValueError(f"{name} must be row-major with unit column stride")
+    ld = t.stride(0)
+    if t.size(0) <= 1:
+        ld = max(ld, t.size(1))
+    return ld
+
+
+@dataclass
+class CsrMatrix:
+    """CSR matrix on the device; mirrors SpMatCsrDescr_t
+    (test-cg/include/dataloader/dataloader.hpp:146-168) with int32 indices and
+    nullable values (None = all ones, as __guard_load_default_one)."""
+
+    nrow: int
+    ncol: int
+    rowptr: torch.Tensor
+    colind: torch.Tensor
+    vals: Optional[torch.Tensor] = None
+
+    @property
+    def nnz(self) -> int:
+        return int(self.colind.numel())
+
+    @classmethod
+    def from_numpy(cls, nrow, ncol, rowptr, colind, vals=None, device="cuda") -> "CsrMatrix":
+        rp = torch.as_tensor(rowptr, dtype=torch.int32).to(device)
+        ci = torch.as_tensor(colind, dtype=torch.int32).to(device)
+        va = None if vals is None else torch.as_tensor(vals, dtype=torch.float32).to(device)
+        return cls(int(nrow), int(ncol), rp.contiguous(), ci.contiguous(), None if va is None else va.contiguous())
+
+    def desc(self) -> _abi.CsrDesc:
+        d = _abi.CsrDesc()
+        d.nrow, d.ncol, d.nnz = self.nrow, self.ncol, self.nnz
+        d.rowptr = self.rowptr.data_ptr() if self.rowptr.numel() else None
+        d.colind = self.colind.data_ptr() if self.colind.numel() else None
+        d.vals = None if self.vals is None or self.vals.numel() == 0 else self.vals.data_ptr()
+        return d
+
+
+# ---- SpMM --------------------------------------------------------------------------------
+
+def spmm_describe(A: CsrMatrix, N: int, dtype=torch.float32, variant="auto", knobs=None) -> dict:
+    lib = _abi.load()
+    info = _abi.LaunchInfo()
+    v = _abi.SPMM_VARIANTS[variant] if isinstance(variant, str) else int(variant)
+    _abi.check(lib.segscan_spmm_describe(byref(A.desc()), N, _DTYPES[dtype], v, _abi.knobs_ptr(knobs), byref(info)),
+               "spmm_describe")
+    return info.as_dict()
+
+
+def spmm_workspace(A: CsrMatrix, N: int, dtype=torch.float32, variant="auto", knobs=None, device=None) -> torch.Tensor:
+    info = spmm_describe(A, N, dtype, variant, knobs)
+    return torch.empty(max(1, info["workspace_bytes"]), dtype=torch.uint8, device=device or A.rowptr.device)
+
+
+def spmm(A: CsrMatrix, B: torch.Tensor, variant="auto", knobs=None, out: Optional[torch.Tensor] = None,
+         accumulate: bool = False, workspace: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+    """C (=|+=) A @ B with B of dtype float32/bfloat16/float16 and fp32 C."""
+    _require_device(A.rowptr, B)
+    lib = _abi.load()
+    N = B.size(1)
+    if B.size(0) != A.ncol:
+        raise ValueError(f"B has {B.size(0)} rows, A has {A.ncol} columns")
+    if out is None:
+        out = torch.empty((A.nrow, N), dtype=torch.float32, device=B.device)
+    if out.dtype != torch.float32 or tuple(out.shape) != (A.nrow, N):
+        raise ValueError("out must be float32 of shape (nrow, N)")
+    v = _abi.SPMM_VARIANTS[variant] if isinstance(variant, str) else int(variant)
+    kp = _abi.knobs_ptr(knobs)
+    desc = A.desc()
+    if workspace is None:
+        ws_bytes = c_size_t(0)
+        _abi.check(lib.segscan_spmm_workspace_size(byref(desc), N, _dtype_code(B), v, kp, byref(ws_bytes)),
+                   "spmm_workspace_size")
+        workspace = torch.empty(max(1, ws_bytes.value), dtype=torch.uint8, device=B.device)
+    st = lib.segscan_spmm_csr(byref(desc), N, _ptr(B), _leading_dim(B, "B"), _dtype_code(B), _ptr(out),
+                              _leading_dim(out, "C"), v, kp, _abi.ACCUMULATE if accumulate else 0,
+                              _ptr(workspace), workspace.numel(), _stream(stream))
+    _abi.check(st, "spmm")
+    return out
+
+
+# ---- segment_coo ---------------------------------------------------------------------------
+
+def segment_describe(E: int, N: int, dst_len: int, dtype=torch.float32, variant="auto", knobs=None) -> dict:
+    lib = _abi.load()
+    info = _abi.LaunchInfo()
+    v = _abi.SEG_VARIANTS[variant] if isinstance(variant, str) else int(variant)
+    _abi.check(lib.segscan_segment_coo_describe(E, N, dst_len, _DTYPES[dtype], v, _abi.knobs_ptr(knobs), byref(info)),
+               "segment_coo_describe")
+    return info.as_dict()
+
+
+def segment_coo(src: torch.Tensor, index: torch.Tensor, dst_len: int, variant="auto", knobs=None,
+                out: Optional[torch.Tensor] = None, accumulate: bool = False,
+                workspace: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+    """dst[s] (=|+=) sum of src rows e with index[e] == s (index sorted)."""
+    _require_device(src, index)
+    lib = _abi.load()
+    E, N = src.shape
+    if index.dtype != torch.int32 or index.numel() != E:
+        raise ValueError("index must be int32 with one entry per src row")
+    if out is None:
+        out = torch.empty((dst_len, N), dtype=torch.float32, device=src.device)
+    v = _abi.SEG_VARIANTS[variant] if isinstance(variant, str) else int(variant)
+    kp = _abi.knobs_ptr(knobs)
+    if workspace is None:
+        ws_bytes = c_size_t(0)
+        _abi.check(lib.segscan_segment_coo_workspace_size(E, N, dst_len, _dtype_code(src), v, kp, byref(ws_bytes)),
+                   "segment_coo_workspace_size")
+        workspace = torch.empty(max(1, ws_bytes.value), dtype=torch.uint8, device=src.device)
+    st = lib.segscan_segment_coo(_ptr(src), _leading_dim(src, "src"), _dtype_code(src), _ptr(index), E, N, _ptr(out),
+                                 _leading_dim(out, "dst"), dst_len, v, kp, _abi.ACCUMULATE if accumulate else 0,
+                                 _ptr(workspace), workspace.numel(), _stream(stream))
+    _abi.check(st, "segment_coo")
+    return out
+
+
+# ---- backward pieces ----------------------------------------------------------------------
+
+@dataclass
+class CscMatrix:
+    """Transpose of a CsrMatrix as a CSR matrix of A^T (ncol x nrow) plus perm:
+    perm[q] = CSR position of transposed entry q."""
+
+    t: CsrMatrix
+    perm: torch.Tensor
+
+
+def csr_transpose(A: CsrMatrix, stream=None) -> CscMatrix:
+    _require_device(A.rowptr)
+    lib = _abi.load()
+    dev = A.rowptr.device
+    colptr = torch.empty(A.ncol + 1, dtype=torch.int32, device=dev)
+    rowind = torch.empty(max(A.nnz, 1), dtype=torch.int32, device=dev)
+    perm = torch.empty(max(A.nnz, 1), dtype=torch.int32, device=dev)
+    vals_t = torch.empty(max(A.nnz, 1), dtype=torch.float32, device=dev)
+    desc = A.desc()
+    ws_bytes = c_size_t(0)
+    _abi.check(lib.segscan_csr_transpose_workspace_size(byref(desc), byref(ws_bytes)), "transpose_workspace_size")
+    ws = torch.empty(max(1, ws_bytes.value), dtype=torch.uint8, device=dev)
+    _abi.check(lib.segscan_csr_transpose(byref(desc), _ptr(colptr), _ptr(rowind), _ptr(perm), _ptr(vals_t), _ptr(ws),
+                                         ws.numel(), _stream(stream)), "csr_transpose")
+    n = A.nnz
+    t = CsrMatrix(A.ncol, A.nrow, colptr, rowind[:n], None if A.vals is None else vals_t[:n])
+    return CscMatrix(t, perm[:n])
+
+
+def sddmm(A: CsrMatrix, X: torch.Tensor, Y: torch.Tensor, out: Optional[torch.Tensor] = None, stream=None):
+    """out[p] = <X[row(p)], Y[colind[p]]> on A's pattern (fp32 result)."""
+    _require_device(X, Y)
+    lib = _abi.load()
+    if X.dtype != Y.dtype:
+        raise TypeError("X and Y must share a dtype")
+    N = X.size(1)
+    if out is None:
+        out = torch.empty(max(A.nnz, 1), dtype=torch.float32, device=X.device)[: A.nnz]
+    _abi.check(lib.segscan_sddmm_csr(byref(A.desc()), N, _ptr(X), _leading_dim(X, "X"), _ptr(Y),
+                                     _leading_dim(Y, "Y"), _dtype_code(X), _ptr(out), _stream(stream)), "sddmm")
+    return out
+
+
+def segment_counts(index: torch.Tensor, dst_len: int, stream=None) -> torch.Tensor:
+    _require_device(index)
+    lib = _abi.load()
+    counts = torch.empty(max(dst_len, 1), dtype=torch.int32, device=index.device)[:dst_len]
+    _abi.check(lib.segscan_segment_counts(_ptr(index), index.numel(), dst_len, _ptr(counts), _stream(stream)),
+               "segment_counts")
+    return counts
+
+
+def segment_gather(ddst: torch.Tensor, index: torch.Tensor, counts: Optional[torch.Tensor] = None,
+                   out: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+    """dsrc[e] = ddst[index[e]] (divided by the segment count when counts is given)."""
+    _require_device(ddst, index)
+    lib = _abi.load()
+    E = index.numel()
+    N = ddst.size(1)
+    if out is None:
+        out = torch.empty((E, N), dtype=torch.float32, device=ddst.device)
+    _abi.check(lib.segscan_segment_gather(_ptr(ddst), _leading_dim(ddst, "ddst"), _ptr(index), E, N, _ptr(counts),
+                                          _ptr(out), _leading_dim(out, "dsrc"), _stream(stream)), "segment_gather")
+    return out
+
+
+def check_csr(A: CsrMatrix, stream=None) -> int:
+    """Device-side structure check; returns the flag bits (0 = well formed)."""
+    lib = _abi.load()
+    flags = torch.zeros(1, dtype=torch.int32, device=A.rowptr.device)
+    _abi.check(lib.segscan_check_csr(byref(A.desc()), _ptr(flags), _stream(stream)), "check_csr")
+    return int(flags.item())
+
+
+def check_index(index: torch.Tensor, dst_len: int, stream=None) -> int:
+    lib = _abi.load()
+    flags = torch.zeros(1, dtype=torch.int32, device=index.device)
+    _abi.check(lib.segscan_check_index(_ptr(index), index.numel(), dst_len, _ptr(flags), _stream(stream)),
+               "check_index")
+    return int(flags.item())

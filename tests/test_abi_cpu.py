@@ -1,0 +1,177 @@
+"""CPU-side checks of the C ABI: the library loads without a GPU, exports every
+function include/segscan.h declares, and rejects bad arguments on the host
+before any HIP call (every call here fails validation, so none reaches the
+device)."""
+import ctypes
+import re
+from ctypes import byref, c_size_t
+from pathlib import Path
+
+import pytest
+
+from segscan_amd import _abi
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "segscan.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(segscan_[a-z_0-9]+)\s*\(", text, flags=re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return _abi.load()
+
+
+def test_header_declares_functions():
+    names = declared_functions()
+    assert "segscan_spmm_csr" in names and "segscan_segment_coo" in names
+    assert len(names) >= 16
+
+
+def test_library_exports_every_declared_symbol(lib):
+    for name in declared_functions():
+        assert hasattr(lib, name), f"{name} declared in segscan.h but not exported"
+    # and the binding knows every declared function
+    assert set(declared_functions()) == set(_abi.EXPORTED)
+
+
+def test_version_and_status_strings(lib):
+    assert lib.segscan_abi_version() == 1
+    for s in range(5):
+        assert lib.segscan_status_string(s).decode().startswith("SEGSCAN_")
+
+
+def _desc(nrow=10, ncol=10, nnz=20, rowptr=1, colind=1, vals=None):
+    d = _abi.CsrDesc()
+    d.nrow, d.ncol, d.nnz = nrow, ncol, nnz
+    d.rowptr, d.colind, d.vals = rowptr, colind, vals
+    return d
+
+
+def _spmm(lib, desc, N=8, B=0x1000, ldb=8, C=0x2000, ldc=8, variant=_abi.SPMM_EBSR, knobs=None, flags=0,
+          ws=None, ws_bytes=0, dtype=_abi.F32):
+    kp = None if knobs is None else byref(_abi.Knobs.make(**knobs))
+    return lib.segscan_spmm_csr(None if desc is None else byref(desc), N, B, ldb, dtype, C, ldc, variant, kp, flags,
+                                ws, ws_bytes, None)
+
+
+@pytest.mark.parametrize("case,expect", [
+    (dict(desc=None), _abi.EINVAL),
+    (dict(desc=_desc(nrow=-1)), _abi.EINVAL),
+    (dict(desc=_desc(rowptr=None)), _abi.EINVAL),
+    (dict(desc=_desc(colind=None)), _abi.EINVAL),
+    (dict(desc=_desc(nrow=0, nnz=5)), _abi.EINVAL),
+    (dict(desc=_desc(nnz=2**31)), _abi.EINVAL),
+    (dict(desc=_desc(), N=-1), _abi.EINVAL),
+    (dict(desc=_desc(), B=None), _abi.EINVAL),
+    (dict(desc=_desc(), C=None), _abi.EINVAL),
+    (dict(desc=_desc(), ldb=4), _abi.EINVAL),
+    (dict(desc=_desc(), ldc=7), _abi.EINVAL),
+    (dict(desc=_desc(), C=0x2002), _abi.EINVAL),
+    (dict(desc=_desc(), dtype=7), _abi.EINVAL),
+    (dict(desc=_desc(), flags=4), _abi.EINVAL),
+    (dict(desc=_desc(), variant=42), _abi.EINVAL),
+    (dict(desc=_desc(), knobs=dict(group_factor=7)), _abi.EUNSUPPORTED),
+    (dict(desc=_desc(), knobs=dict(thread_per_block=2048)), _abi.EUNSUPPORTED),
+    (dict(desc=_desc(), knobs=dict(coarsen_factor=3)), _abi.EUNSUPPORTED),
+    (dict(desc=_desc(), knobs=dict(block_numer=-1, block_denom=2)), _abi.EINVAL),
+    (dict(desc=_desc(), knobs=dict(write_policy=9)), _abi.EINVAL),
+    (dict(desc=_desc(), variant=_abi.SPMM_EBPR, knobs=dict(group_factor=6, tile_factor=5)), _abi.EUNSUPPORTED),
+    (dict(desc=_desc(), variant=_abi.SPMM_EBPR, knobs=dict(tile_factor=7)), _abi.EUNSUPPORTED),
+    (dict(desc=_desc(), variant=_abi.SPMM_EBPR, knobs=dict(coarsen_factor=8)), _abi.EUNSUPPORTED),
+    (dict(desc=_desc(), variant=_abi.SPMM_RBSR, knobs=dict(tile_factor=9)), _abi.EUNSUPPORTED),
+    # owner policy needs a workspace
+    (dict(desc=_desc(), knobs=dict(write_policy=2)), _abi.EWORKSPACE),
+    (dict(desc=_desc(), knobs=dict(write_policy=2), ws=0x3000, ws_bytes=1), _abi.EWORKSPACE),
+])
+def test_spmm_validation(lib, case, expect):
+    st = _spmm(lib, **case)
+    assert st == expect, lib.segscan_last_error().decode()
+    assert lib.segscan_last_error().decode() != ""
+
+
+def test_segment_validation(lib):
+    k = None
+    f = lib.segscan_segment_coo
+    assert f(None, 4, 0, 0x10, 10, 4, 0x20, 4, 5, 0, k, 0, None, 0, None) == _abi.EINVAL  # src NULL
+    assert f(0x10, 4, 0, None, 10, 4, 0x20, 4, 5, 0, k, 0, None, 0, None) == _abi.EINVAL  # index NULL
+    assert f(0x10, 4, 0, 0x10, 10, 4, None, 4, 5, 0, k, 0, None, 0, None) == _abi.EINVAL  # dst NULL
+    assert f(0x10, 2, 0, 0x10, 10, 4, 0x20, 4, 5, 0, k, 0, None, 0, None) == _abi.EINVAL  # ld_src < N
+    assert f(0x10, 4, 0, 0x10, 10, 4, 0x20, 4, 0, 0, k, 0, None, 0, None) == _abi.EINVAL  # dst_len 0
+    assert f(0x10, 4, 0, 0x10, -1, 4, 0x20, 4, 5, 0, k, 0, None, 0, None) == _abi.EINVAL  # E < 0
+    assert f(0x10, 4, 0, 0x10, 10, 4, 0x20, 4, 5, 99, k, 0, None, 0, None) == _abi.EINVAL  # variant
+    kk = _abi.Knobs.make(write_policy=2)
+    assert f(0x10, 4, 0, 0x10, 10, 4, 0x20, 4, 5, _abi.SEG_SR, byref(kk), 0, None, 0, None) == _abi.EWORKSPACE
+    kk = _abi.Knobs.make(dtile_size=3)
+    assert f(0x10, 4, 0, 0x10, 10, 4, 0x20, 4, 5, _abi.SEG_SORTED_SR, byref(kk), 0, None, 0, None) == _abi.EUNSUPPORTED
+
+
+def test_backward_validation(lib):
+    d = _desc()
+    assert lib.segscan_csr_transpose(byref(d), None, 0x10, 0x10, None, 0x10, 1 << 20, None) == _abi.EINVAL
+    assert lib.segscan_csr_transpose(byref(d), 0x10, 0x10, 0x10, None, None, 0, None) == _abi.EWORKSPACE
+    assert lib.segscan_sddmm_csr(byref(d), 8, None, 8, 0x10, 8, 0, 0x10, None) == _abi.EINVAL
+    assert lib.segscan_segment_gather(None, 8, 0x10, 10, 8, None, 0x10, 8, None) == _abi.EINVAL
+    assert lib.segscan_segment_counts(None, 10, 5, 0x10, None) == _abi.EINVAL
+    assert lib.segscan_check_csr(byref(d), None, None) == _abi.EINVAL
+
+
+def _describe(lib, nrow, nnz, N, variant, dtype=_abi.F32, **knobs):
+    info = _abi.LaunchInfo()
+    d = _desc(nrow=nrow, ncol=nrow, nnz=nnz)
+    kp = byref(_abi.Knobs.make(**knobs)) if knobs else None
+    st = lib.segscan_spmm_describe(byref(d), N, dtype, variant, kp, byref(info))
+    assert st == _abi.OK, lib.segscan_last_error().decode()
+    return info
+
+
+def test_describe_plain_ebpr_geometry(lib):
+    # spmm.cuh:172-187 at N=64: CF 4, 8 panels x 32 lanes, gridDim.y 2
+    info = _describe(lib, 262144, 4194304, 64, _abi.SPMM_PLAIN_EBPR)
+    assert (info.group_size, info.tile_size, info.coarsen_factor) == (32, 32, 4)
+    assert info.thread_per_block == 256 and info.grid_y == 2
+    assert info.write_policy == _abi.WRITE_ATOMIC
+    assert info.nchunks == 4194304 // 32  # tiles of 32 nnz, one partition entry each
+
+
+def test_describe_cg_defaults_and_knobs(lib):
+    info = _describe(lib, 262144, 4194304, 64, _abi.SPMM_EBSR)  # (4,256,4,1,1,16)
+    assert (info.group_size, info.coarsen_factor, info.thread_nz) == (16, 4, 1)
+    assert info.grid_y == 1
+    info = _describe(lib, 262144, 4194304, 64, _abi.SPMM_EBSR, group_factor=4, thread_nz=16, write_policy=2)
+    assert info.chunk == 256 and info.nchunks == 4194304 // 256
+    assert info.workspace_bytes >= info.nchunks * (4 + 4 + 4 + 64 * 4)
+    info = _describe(lib, 262144, 4194304, 64, _abi.SPMM_RBPR)  # (3,256,3,1,2): G == T -> stores
+    assert info.write_policy == _abi.WRITE_OWNER and info.workspace_bytes == 0
+    info = _describe(lib, 262144, 4194304, 64, _abi.SPMM_RBPR, group_factor=2, tile_factor=3)
+    assert info.write_policy == _abi.WRITE_ATOMIC
+    # bf16 storage doubles the default columns per lane (128-bit loads)
+    info = _describe(lib, 262144, 4194304, 64, _abi.SPMM_PLAIN_EBPR, dtype=_abi.BF16)
+    assert info.coarsen_factor == 8
+
+
+def test_describe_narrows_oversized_slices(lib):
+    # tile 64 x one-column panels would need 64 x 64 lanes: the slice shrinks to 16 columns
+    info = _describe(lib, 1000, 5000, 64, _abi.SPMM_EBPR, group_factor=6, tile_factor=6, coarsen_factor=1)
+    assert info.thread_per_block == 1024 and info.grid_y == 4
+
+
+def test_describe_auto_is_concrete(lib):
+    info = _describe(lib, 262144, 4194304, 64, _abi.SPMM_AUTO)
+    assert info.variant in range(4)
+    assert info.thread_per_block % 64 == 0 and info.thread_per_block <= 1024
+
+
+def test_workspace_size_query(lib):
+    d = _desc(nrow=1000, ncol=1000, nnz=5000)
+    n = c_size_t(0)
+    assert lib.segscan_spmm_workspace_size(byref(d), 16, _abi.F32, _abi.SPMM_PLAIN_RBSR, None, byref(n)) == _abi.OK
+    assert n.value == 0
+    assert lib.segscan_spmm_workspace_size(byref(d), 16, _abi.F32, _abi.SPMM_EBPR, None, byref(n)) == _abi.OK
+    assert n.value > 0
+    assert lib.segscan_csr_transpose_workspace_size(byref(d), byref(n)) == _abi.OK
+    assert n.value >= 1000 * 4
