@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Benchmark of the SpMM hot path on one MI355X.
+
+Workload: power-law CSR A, 262144 x 262144, row degrees Zipf(2.1) clipped to
+[1, 20000] and scaled to ~4.19M nnz, columns uniform and sorted, values
+U(-1, 1), dense B 262144 x 64 U(-1, 1), seed 0 (synthetic data).
+
+One step = one autotuned fp32 SpMM C = A.B (partition, main kernel and carry
+fix-up; inputs resident in HBM). `value` is the whole step's algorithmic
+(compulsory-byte) bandwidth:
+    B_alg = 4 (M + 1) + nnz (4 + 4) + K_used N 4 + M N 4      (bytes)
+The same line reports the plain fp32 kernels (the reference's fixed
+configurations), the autotuned bf16-B SpMM, fwd+bwd time, the roofline of the
+main kernel (HIP events recorded around it on its stream) and the reference's
+CPU check path (the oracle restatement) timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+# resolved variant id (segscan_spmm_variant_t) -> main kernel name in rocprofv3 traces
+KERNEL_OF_VARIANT = {0: "pr_eb_kernel", 1: "pr_rb_kernel", 2: "sr_kernel", 3: "rbsr_kernel",
+                     4: "pr_eb_kernel", 5: "pr_rb_kernel", 6: "sr_kernel", 7: "rbsr_kernel"}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--N", type=int, default=64)
+    ap.add_argument("--retune", action="store_true", help="ignore the committed tuning table")
+    ap.add_argument("--tune-reps", type=int, default=7)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only-tuned", action="store_true", help="skip the comparison legs (for profiling)")
+    ap.add_argument("--dump", type=str, default=None, help="write tuning results to this JSON file")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this single-GPU build "
+              "(the row-block sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from segscan_amd import _abi, autotune, ops, synth
+
+    dev = torch.device("cuda:0")
+    N = args.N
+    t0 = time.time()
+    m = synth.powerlaw_csr(seed=0)
+    stats = autotune.row_stats(m.rowptr)
+    log(f"workload: {m.nrow}x{m.ncol} nnz={m.nnz} max_row={stats['max']} N={N} ({time.time() - t0:.1f}s)")
+    A = m.to_device(dev)
+    rng = np.random.default_rng(0)
+    B32 = torch.from_numpy(rng.uniform(-1, 1, size=(m.ncol, N)).astype(np.float32)).to(dev)
+    Bbf = B32.to(torch.bfloat16)
+    C = torch.empty((m.nrow, N), dtype=torch.float32, device=dev)
+    k_used = int(np.count_nonzero(np.bincount(m.colind, minlength=m.ncol)))
+
+    def b_alg(s_b: int) -> int:
+        return 4 * (m.nrow + 1) + m.nnz * 8 + k_used * N * s_b + m.nrow * N * 4
+
+    def b_gather(s_b: int) -> int:
+        return 4 * (m.nrow + 1) + m.nnz * 8 + m.nnz * N * s_b + m.nrow * N * 4
+
+    # ---- tuning (untimed setup) ----
+    table = {} if args.retune else autotune.load_table()
+    tuned = {}
+    dump = {}
+    for dtype, B in ((torch.float32, B32), (torch.bfloat16, Bbf)):
+        key = autotune.table_key("spmm", dtype, N, stats)
+        hit = autotune.lookup(table, key)
+        if hit is not None:
+            tuned[dtype] = hit
+            log(f"tuning table hit {key}: {hit}")
+            continue
+        cands = autotune.spmm_candidates(N, dtype, stats)
+        t1 = time.time()
+        res = autotune.tune_spmm(A, B, cands, reps=args.tune_reps)
+        log(f"tuned {key} over {len(cands)} candidates in {time.time() - t1:.1f}s: best {res[0]['ms']:.4f} ms "
+            f"{res[0]['variant']} {res[0]['knobs']}")
+        tuned[dtype] = (res[0]["variant"], res[0]["knobs"])
+        dump[key] = res
+    if args.dump:
+        Path(args.dump).write_text(json.dumps(dump, indent=1, default=str))
+
+    lib = _abi.load()
+
+    def make_step(B, variant, knobs, out):
+        ws = ops.spmm_workspace(A, N, B.dtype, variant, knobs)
+
+        def step():
+            ops.spmm(A, B, variant=variant, knobs=knobs, out=out, workspace=ws)
+        return step
+
+    def timed(step, steps, warmup, kernel_events=True):
+        """Wall time of `steps` steps bracketed by synchronize, plus the mean
+        duration of the main kernel from per-step HIP events."""
+        for _ in range(warmup):
+            step()
+        pairs = []
+        if kernel_events:
+            for _ in range(steps):
+                s = torch.cuda.Event(enable_timing=True)
+                e = torch.cuda.Event(enable_timing=True)
+                s.record()
+                e.record()
+                pairs.append((s, e))
+        torch.cuda.synchronize()
+        t_start = torch.cuda.Event(enable_timing=True)
+        t_stop = torch.cuda.Event(enable_timing=True)
+        t_start.record()
+        for i in range(steps):
+            if kernel_events:
+                lib.segscan_set_timing_events(pairs[i][0].cuda_event, pairs[i][1].cuda_event)
+            step()
+        lib.segscan_set_timing_events(None, None)
+        t_stop.record()
+        torch.cuda.synchronize()
+        total_ms = t_start.elapsed_time(t_stop)
+        kern_ms = float(np.mean([s.elapsed_time(e) for s, e in pairs])) if kernel_events else None
+        return total_ms / steps, kern_ms
+
+    # ---- headline: tuned fp32 ----
+    v32, k32 = tuned[torch.float32]
+    step32 = make_step(B32, v32, k32, C)
+    torch.cuda.synchronize()
+    ms_step, kern_ms = timed(step32, args.steps, args.warmup)
+    value = b_alg(4) / (ms_step * 1e-3) / 1e9
+    log(f"tuned fp32: {ms_step * 1e3:.1f} us/step, main kernel {kern_ms * 1e3:.1f} us, {value:.0f} GB/s")
+    info32 = ops.spmm_describe(A, N, torch.float32, v32, k32)
+
+    extra = {}
+    if not args.only_tuned:
+        # plain fp32 kernels (the reference's fixed launch configurations)
+        plain = {}
+        for pv in ("plain_ebpr", "plain_rbpr", "plain_ebsr", "plain_rbsr"):
+            ms, km = timed(make_step(B32, pv, None, C), max(5, args.steps // 5), 3)
+            plain[pv] = {"ms": round(ms, 5), "kernel_ms": round(km, 5), "GBps": round(b_alg(4) / (ms * 1e-3) / 1e9, 1)}
+        # tuned bf16-B
+        vbf, kbf = tuned[torch.bfloat16]
+        ms_bf, km_bf = timed(make_step(Bbf, vbf, kbf, C), args.steps, args.warmup)
+        # fwd + bwd: C = A.B; dB = A^T.dC; dvals = SDDMM(dC, B) on A's pattern
+        dC = torch.from_numpy(rng.standard_normal((m.nrow, N)).astype(np.float32)).to(dev)
+        T = ops.csr_transpose(A)
+        dB = torch.empty_like(B32)
+        dvals = torch.empty(m.nnz, dtype=torch.float32, device=dev)
+        ws_t = ops.spmm_workspace(T.t, N, torch.float32, "auto")
+        ws_f = ops.spmm_workspace(A, N, torch.float32, v32, k32)
+
+        def fwd_bwd():
+            ops.spmm(A, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
+            ops.spmm(T.t, dC, variant="auto", out=dB, workspace=ws_t)
+            ops.sddmm(A, dC, B32, out=dvals)
+
+        def fwd_bwd_with_transpose():
+            ops.spmm(A, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
+            Tn = ops.csr_transpose(A)
+            ops.spmm(Tn.t, dC, variant="auto", out=dB, workspace=ws_t)
+            ops.sddmm(A, dC, B32, out=dvals)
+
+        fb_ms, _ = timed(fwd_bwd, 10, 3, kernel_events=False)
+        fbt_ms, _ = timed(fwd_bwd_with_transpose, 5, 2, kernel_events=False)
+        plain_ms = plain["plain_ebpr"]["ms"]
+        extra = {
+            "plain_f32": plain,
+            "tuned_f32": {"variant": v32, "knobs": k32, "ms": round(ms_step, 5), "GBps": round(value, 1)},
+            "tuned_bf16": {"variant": vbf, "knobs": kbf, "ms": round(ms_bf, 5), "kernel_ms": round(km_bf, 5),
+                           "GBps_alg": round(b_alg(2) / (ms_bf * 1e-3) / 1e9, 1),
+                           "speedup_vs_plain_ebpr_f32": round(plain_ms / ms_bf, 3),
+                           "speedup_vs_tuned_f32": round(ms_step / ms_bf, 3)},
+            "fwd_bwd_ms": round(fb_ms, 4),
+            "fwd_bwd_ms_incl_transpose": round(fbt_ms, 4),
+            "gather_model_GBps_tuned_f32": round(b_gather(4) / (ms_step * 1e-3) / 1e9, 1),
+        }
+        log(f"plain: {plain}")
+        log(f"bf16 tuned {ms_bf * 1e3:.1f} us; fwd+bwd {fb_ms:.3f} ms ({fbt_ms:.3f} ms incl. transpose)")
+
+    # ---- CPU baseline: the reference's host check path (oracle restatement) ----
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        Bh = B32.cpu().numpy()
+        Ch = np.zeros((m.nrow, N), np.float32)
+        oracle.spmm_f32(m.rowptr, m.colind, m.vals, Bh, C=Ch)  # warm-up
+        times = []
+        for _ in range(25):
+            Ch[:] = 0
+            t1 = time.perf_counter()
+            oracle.spmm_f32(m.rowptr, m.colind, m.vals, Bh, C=Ch)
+            times.append(time.perf_counter() - t1)
+        cpu_s = float(np.median(times))
+        cpu = {"value": round(b_alg(4) / cpu_s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+               "ms": round(cpu_s * 1e3, 1),
+               "sample": "full workload (262144x262144, %d nnz, N=%d), fp32 i->p->j loop, 1 warm-up + median of 25"
+                         % (m.nnz, N)}
+        log(f"cpu baseline: {cpu_s * 1e3:.1f} ms ({cpu['value']} GB/s), host {os.cpu_count()} cpus")
+
+    achieved = b_alg(4) / (kern_ms * 1e-3) / 1e9
+    line = {
+        "metric": "SpMM GB/s (% HBM peak) + fwd+bwd ms, N=64: tuned bf16 vs plain fp32",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "powerlaw_spmm_262144x262144_zipf2.1_N64", "nrow": m.nrow, "ncol": m.ncol,
+                   "nnz": m.nnz, "N": N, "seed": 0, "variant": v32, "knobs": k32,
+                   "launch": {k: info32[k] for k in ("group_size", "coarsen_factor", "thread_nz", "chunk",
+                                                      "thread_per_block", "grid_x", "grid_y", "write_policy")}},
+        "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": KERNEL_OF_VARIANT[info32["variant"]],
+                     "kernel_us": round(kern_ms * 1e3, 2), "algorithmic_bytes": b_alg(4)},
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

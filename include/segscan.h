@@ -219,6 +219,13 @@ segscan_status_t segscan_check_csr(const segscan_csr_t* A, int32_t* flags_dev, v
 segscan_status_t segscan_check_index(const int32_t* index, int64_t E, int64_t dst_len, int32_t* flags_dev,
                                      void* stream);
 
+/* ---- measurement hook ---------------------------------------------------------------------
+ * With non-NULL hipEvent_t handles, every later segscan_spmm_csr / segscan_segment_coo
+ * call on this host thread records `start` on its stream right before its main kernel
+ * and `stop` right after it, so the pair times that kernel alone (no zero fill,
+ * partition or fix-up pass). NULL, NULL turns the hook off. */
+void segscan_set_timing_events(void* start, void* stop);
+
 /* ---- misc -------------------------------------------------------------------------------- */
 const char* segscan_status_string(segscan_status_t status);
 const char* segscan_last_error(void);

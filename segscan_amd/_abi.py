@@ -109,6 +109,7 @@ _SIGNATURES = {
     "segscan_segment_counts": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
     "segscan_check_csr": (c_int32, [POINTER(CsrDesc), c_void_p, c_void_p]),
     "segscan_check_index": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
+    "segscan_set_timing_events": (None, [c_void_p, c_void_p]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -18,6 +18,15 @@ namespace {
 using namespace segscan;
 
 thread_local std::string g_last_error;
+thread_local hipEvent_t g_ev_start = nullptr;
+thread_local hipEvent_t g_ev_stop = nullptr;
+
+void mark_start(hipStream_t s) {
+  if (g_ev_start) (void)hipEventRecord(g_ev_start, s);
+}
+void mark_stop(hipStream_t s) {
+  if (g_ev_stop) (void)hipEventRecord(g_ev_stop, s);
+}
 
 segscan_status_t fail(segscan_status_t st, const char* fmt, ...) {
   char buf[512];
@@ -565,6 +574,11 @@ extern "C" {
 
 int32_t segscan_abi_version(void) { return 1; }
 
+void segscan_set_timing_events(void* start, void* stop) {
+  g_ev_start = static_cast<hipEvent_t>(start);
+  g_ev_stop = static_cast<hipEvent_t>(stop);
+}
+
 const char* segscan_status_string(segscan_status_t s) {
   switch (s) {
     case SEGSCAN_OK: return "SEGSCAN_OK";
@@ -679,7 +693,9 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N, const void*
         e = launch_partition_rows(A->rowptr, p.M, nnz, pl.T, pl.ntiles, const_cast<int32_t*>(p.tile_row), nullptr, s);
         if (e != hipSuccess) return hip_status(e, "partition");
       }
+      mark_start(s);
       e = launch_spmm_pr(vt, pl.CF, pl.G, p, l, pl.fam == kEbpr, compiled);
+      mark_stop(s);
       break;
     }
     case kEbsr: {
@@ -710,7 +726,9 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N, const void*
                                 const_cast<int32_t*>(p.chunk_prev), s);
       if (e != hipSuccess) return hip_status(e, "partition");
       l.shmem = static_cast<size_t>(pl.tpb) * 12;
+      mark_start(s);
       e = launch_spmm_sr(vt, pl.CF, pl.G, p, l, compiled);
+      mark_stop(s);
       if (e == hipSuccess && compiled && pl.policy == SEGSCAN_WRITE_OWNER)
         e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, C, ldc, s);
       break;
@@ -731,7 +749,9 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N, const void*
       p.slice = pl.slice;
       p.rows_total = M;
       p.accumulate = accumulate;
+      mark_start(s);
       e = launch_spmm_rbsr(vt, pl.CF, p, l, compiled);
+      mark_stop(s);
       break;
     }
   }
@@ -832,7 +852,9 @@ segscan_status_t segscan_segment_coo(const void* src, int64_t ld_src, segscan_dt
     p.slice = pl.slice;
     p.ntiles = pl.ntiles;
     p.accumulate = accumulate;
+    mark_start(s);
     e = launch_seg_pr(vt, pl.CF, pl.G, p, l, compiled);
+    mark_stop(s);
   } else {
     SrParams p{};
     p.M = static_cast<int>(dst_len);
@@ -854,7 +876,9 @@ segscan_status_t segscan_segment_coo(const void* src, int64_t ld_src, segscan_dt
       p.carry_val = reinterpret_cast<float*>(ws + pl.off_cval);
     }
     l.shmem = static_cast<size_t>(pl.tpb) * 12;
+    mark_start(s);
     e = launch_seg_sr(vt, pl.CF, pl.G, p, l, compiled);
+    mark_stop(s);
     if (e == hipSuccess && compiled && pl.policy == SEGSCAN_WRITE_OWNER)
       e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, dst, ld_dst, s);
   }
