@@ -48,11 +48,29 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-tuned", action="store_true", help="skip the comparison legs (for profiling)")
     ap.add_argument("--dump", type=str, default=None, help="write tuning results to this JSON file")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB fp32 device-to-device 5 times (scales HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch give the
+    factor between the counters and real bytes (256 MiB read + 256 MiB written)."""
+    import torch
+    n = 64 * 1024 * 1024
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
 
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this single-GPU build "
               "(the row-block sharding design is described in DESIGN.md)", file=sys.stderr)
