@@ -56,6 +56,31 @@ __device__ __forceinline__ void load_cols(const VT* __restrict__ p, float (&out)
   }
 }
 
+// A lane's CF columns kept as raw storage (one or two 16-byte registers groups),
+// so that many dense-row loads can be in flight before any is widened.
+template <typename VT, int CF>
+struct RawCols {
+  static constexpr int BYTES = CF * static_cast<int>(sizeof(VT));
+  static constexpr int PART = BYTES < 16 ? BYTES : 16;
+  static constexpr int NPART = BYTES / PART;
+  using P = typename RawVec<PART>::T;
+  P part[NPART];
+};
+
+template <typename VT, int CF>
+__device__ __forceinline__ void load_raw(const VT* __restrict__ p, RawCols<VT, CF>& r) {
+  const auto* q = reinterpret_cast<const typename RawCols<VT, CF>::P*>(p);
+#pragma unroll
+  for (int i = 0; i < RawCols<VT, CF>::NPART; ++i) r.part[i] = q[i];
+}
+
+template <typename VT, int CF>
+__device__ __forceinline__ void widen(const RawCols<VT, CF>& r, float (&out)[CF]) {
+  const VT* e = reinterpret_cast<const VT*>(r.part);
+#pragma unroll
+  for (int i = 0; i < CF; ++i) out[i] = to_f32(e[i]);
+}
+
 // Loads the first nv (< CF) of a lane's columns element by element (residue panel).
 template <typename VT, int CF>
 __device__ __forceinline__ void load_cols_partial(const VT* __restrict__ p, int nv, float (&out)[CF]) {
@@ -191,6 +216,50 @@ __device__ __forceinline__ int row_of_nnz(const int32_t* __restrict__ rowptr, in
     else hi = mid;
   }
   return hi - 1;
+}
+
+template <int G>
+__device__ __forceinline__ uint64_t group_mask() {
+  if constexpr (G >= 64) return ~0ull;
+  else return (1ull << G) - 1ull;
+}
+
+// Group-cooperative lookup for an aligned group of G lanes walking consecutive
+// nnz of one chunk. `rp` holds rowptr[r_hint + 1 + g_lane] (INT32_MAX past M),
+// loaded ahead of time by the caller so that its latency overlaps other work.
+// Precondition: rowptr[r_hint] <= e for every active lane, e < rowptr[M].
+// Each lane counts the boundaries <= e in the window with a log2(G)-step
+// binary search over the group's lanes; lanes past the window step to the next
+// G boundaries (group-uniform loop, rare when rows average more than a few nnz).
+template <int G>
+__device__ __forceinline__ int group_row_find(const int32_t* __restrict__ rowptr, int M, int r_hint, int rp, int e,
+                                              bool active) {
+  const int lane = __lane_id();
+  const int gbase = lane & ~(G - 1);
+  const int g_lane = lane - gbase;
+  int base = r_hint + 1;
+  int row = r_hint;
+  bool done = !active;
+  while (true) {
+    int pos = 0;
+#pragma unroll
+    for (int s = G / 2; s >= 1; s >>= 1) {
+      const int v = __shfl(rp, gbase + pos + s - 1);
+      if (v <= e) pos += s;
+    }
+    const int last = __shfl(rp, gbase + G - 1);
+    if (pos == G - 1 && last <= e) pos = G;
+    if (!done && pos < G) {
+      row = base + pos - 1;
+      done = true;
+    }
+    const uint64_t pending = __ballot(!done);
+    if (((pending >> gbase) & group_mask<G>()) == 0) break;
+    base += G;
+    const int j = base + g_lane;
+    rp = (j <= M) ? rowptr[j] : INT32_MAX;
+  }
+  return row;
 }
 
 // Wave-cooperative lookup: every lane of the wave must call it (full EXEC).

@@ -8,8 +8,11 @@
 // in one instruction) and walks a chunk of consecutive nnz in order,
 // accumulating in registers and flushing when the row changes. Per tile of G
 // nnz every lane stages one (column, value, row) triple in LDS; the group then
-// reads them back as broadcast 128-bit LDS loads and issues up to 8 dense-row
-// loads before consuming any, to keep several HBM/MALL requests in flight.
+// reads them back as broadcast LDS loads and issues up to 16 dense-row loads
+// before consuming any, while the next tile's column, value and rowptr window
+// are already in flight (one-tile software pipeline), so that each group keeps
+// many MALL/HBM requests outstanding. Rows come from a per-group rowptr window
+// (group_row_find), not from a per-nnz binary search.
 //
 // Write policies:
 //  * atomic: every flushed row is an atomic add (output zero-filled beforehand);
@@ -27,21 +30,24 @@ namespace segscan {
 template <typename VT, int CF, int G, bool SEG>
 __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   extern __shared__ int32_t sr_smem[];
-  int32_t* s_k = sr_smem;              // [tpb] column (SpMM)
+  int32_t* s_k = sr_smem;                                  // [tpb] column (SpMM)
   float* s_v = reinterpret_cast<float*>(sr_smem + p.tpb);  // [tpb] value (SpMM)
-  int32_t* s_r = sr_smem + 2 * p.tpb;  // [tpb] row / segment
+  int32_t* s_r = sr_smem + 2 * p.tpb;                      // [tpb] row / segment
 
-  constexpr int U = (G < 8) ? G : 8;   // dense-row loads in flight per group
+  constexpr int U = (G < 16) ? G : 16;  // dense-row loads issued back to back per batch
   const int L = threadIdx.x;
   const int g = L / G;
   const int g_lane = L - g * G;
+  const int wave_gbase = __lane_id() - g_lane;  // first lane of this group in the wave
   const int groups_per_block = p.tpb / G;
   const int64_t ggid = static_cast<int64_t>(blockIdx.x) * groups_per_block + g;
   const int col0 = blockIdx.y * (G * CF) + g_lane * CF;
   const int nv = min(CF, p.N - col0);
+  const bool full = nv == CF;
   const VT* __restrict__ B = static_cast<const VT*>(p.B);
-  const int gbase = g * G;             // this group's LDS slots
+  const int sbase = g * G;  // this group's LDS slots
   const bool acc_mode = p.accumulate != 0;
+  const bool owner = p.policy == 2;
 
   for (int64_t c = ggid;; c += p.total_groups) {
     // Chunks grow with the lane index; the wave's first lane has the lowest one.
@@ -61,7 +67,6 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
         prev_row = p.chunk_prev[c];
       }
     }
-    const bool owner = p.policy == 2;
     const bool is_carry = owner && chunk_valid && c0 > 0 && prev_row == first_row;
     if (owner && chunk_valid) {
       if (g_lane == 0 && blockIdx.y == 0) p.carry_row[c] = is_carry ? first_row : -1;
@@ -88,66 +93,69 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
       }
     };
 
-    // Row hint for the wave-cooperative lookup: the wave's lowest nnz so far.
-    int wave_r0 = 0;
-    if constexpr (!SEG) wave_r0 = __builtin_amdgcn_readfirstlane(first_row);
-    const int64_t wave_c0 = static_cast<int64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(c))) * p.chunk;
+    // Metadata of the next tile, loaded one tile ahead: column and value of the
+    // lane's nnz, and the rowptr window starting after r_hint (a row no later
+    // than any nnz still to come in the chunk).
+    int r_hint = first_row;
+    int pk = 0, prow = INT32_MAX, prp = INT32_MAX;
+    float pv = 0.0f;
+    auto prefetch = [&](int64_t off_n) {
+      const bool vn = off_n + g_lane < clen;
+      const int64_t en = c0 + off_n + g_lane;
+      if constexpr (SEG) {
+        prow = vn ? p.index[en] : INT32_MAX;
+      } else {
+        pk = vn ? p.colind[en] : 0;
+        pv = vn ? (p.vals ? p.vals[en] : 1.0f) : 0.0f;
+        if (off_n < clen) {
+          const int jj = r_hint + 1 + g_lane;
+          prp = (jj <= p.M) ? p.rowptr[jj] : INT32_MAX;
+        }
+      }
+    };
+    prefetch(0);
 
     const int ntile = static_cast<int>((p.chunk + G - 1) / G);  // wave-uniform
     for (int j = 0; j < ntile; ++j) {
       const int64_t off = static_cast<int64_t>(j) * G;
-      if (__ballot(off < clen) == 0) break;   // no group of this wave has entries left
+      if (__ballot(off < clen) == 0) break;  // no group of this wave has entries left
       const bool valid = off + g_lane < clen;
-      const int64_t e = c0 + off + g_lane;
+      const int cnt = static_cast<int>(max<int64_t>(0, min<int64_t>(G, clen - off)));
       int row;
       if constexpr (SEG) {
-        row = valid ? p.index[e] : INT32_MAX;
-        s_r[L] = row;
+        row = prow;
       } else {
-        const int64_t e_lo = min(wave_c0 + off, p.nnz - 1);
-        const int e_c = static_cast<int>(valid ? e : e_lo);
-        row = wave_row_lookup(p.rowptr, p.M, wave_r0, e_c);
-        wave_r0 = __builtin_amdgcn_readfirstlane(row);
-        s_r[L] = valid ? row : INT32_MAX;
-        s_k[L] = valid ? p.colind[e] : 0;
-        s_v[L] = valid ? (p.vals ? p.vals[e] : 1.0f) : 0.0f;
+        row = group_row_find<G>(p.rowptr, p.M, r_hint, prp, static_cast<int>(c0 + off + g_lane), valid);
+        const int last_row = __shfl(row, wave_gbase + max(cnt, 1) - 1);
+        if (cnt > 0) r_hint = last_row;
+        s_k[L] = pk;
+        s_v[L] = pv;
       }
+      s_r[L] = valid ? row : INT32_MAX;
       // The triples are read back only by lanes of the same wave: order the LDS
       // writes before the reads without a workgroup barrier.
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-      const int cnt = static_cast<int>(max<int64_t>(0, min<int64_t>(G, clen - off)));
       for (int pp = 0; pp < cnt; pp += U) {
-        int kk[U], rr[U];
-        float vv[U];
+        // Issue the whole batch of dense-row loads before consuming any; the
+        // row and value of each entry are read back from LDS only when consumed.
+        RawCols<VT, CF> raw[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          rr[u] = s_r[gbase + pp + u];
-          if constexpr (SEG) {
-            kk[u] = 0;
-            vv[u] = 1.0f;
-          } else {
-            kk[u] = s_k[gbase + pp + u];
-            vv[u] = s_v[gbase + pp + u];
+          if (full && pp + u < cnt) {
+            const int64_t krow = SEG ? (c0 + off + pp + u) : static_cast<int64_t>(s_k[sbase + pp + u]);
+            load_raw<VT, CF>(B + krow * p.ldb + col0, raw[u]);
           }
         }
-        float b[U][CF];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (pp + u < cnt && nv > 0) {
-            const int64_t krow = SEG ? (c0 + off + pp + u) : static_cast<int64_t>(kk[u]);
-            load_cols_any<VT, CF>(B + krow * p.ldb + col0, nv, b[u]);
-          } else {
-#pragma unroll
-            for (int i = 0; i < CF; ++i) b[u][i] = 0.0f;
-          }
-        }
+        // The next tile's metadata travels while this batch's rows arrive.
+        if (pp == 0) prefetch(off + G);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           if (pp + u < cnt) {
-            const int r = rr[u];
+            const int r = s_r[sbase + pp + u];
+            const float v = SEG ? 1.0f : s_v[sbase + pp + u];
             if (r != cur) {
               flush(cur, false);
               if (owner && !acc_mode && nv > 0) {
@@ -157,8 +165,18 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
 #pragma unroll
               for (int i = 0; i < CF; ++i) acc[i] = 0.0f;
             }
+            float b[CF];
+            if (full) {
+              widen<VT, CF>(raw[u], b);
+            } else if (nv > 0) {
+              const int64_t krow = SEG ? (c0 + off + pp + u) : static_cast<int64_t>(s_k[sbase + pp + u]);
+              load_cols_partial<VT, CF>(B + krow * p.ldb + col0, nv, b);
+            } else {
 #pragma unroll
-            for (int i = 0; i < CF; ++i) acc[i] += vv[u] * b[u][i];
+              for (int i = 0; i < CF; ++i) b[i] = 0.0f;
+            }
+#pragma unroll
+            for (int i = 0; i < CF; ++i) acc[i] += v * b[i];
           }
         }
       }
