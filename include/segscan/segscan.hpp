@@ -1,0 +1,280 @@
+// segscan.hpp — C++ host interface with the reference's launcher names and
+// template knob lists, implemented on the C ABI (segscan.h) and the gfx950 HIP
+// kernels of libsegscan_hip.so.
+//
+// Drop-in for the reference's headers:
+//   test-cg/include/spmm/spmm.cuh      csrspmm_parreduce_nnzbalance (:167), csrspmm_parreduce_rowbalance (:310),
+//                                      csrspmm_rowcaching_nnzbalance (:522), csrspmm_seqreduce_rowbalance (:621)
+//   test-cg/include/spmm/spmm_cg.cuh   csrspmm_parreduce_nnzbalance_cg (:176), csrspmm_parreduce_rowbalance_cg (:333),
+//                                      csrspmm_rowcaching_nnzbalance_cg (:553), csrspmm_seqreduce_rowbalance_cg (:664)
+//   new-cg/include/segscan/segscan.cuh segment_coo (:150), segment_coo_new (:327), segment_coo_sr (:817),
+//                                      segment_coo_noshmem_sr (:868), segment_coo_noshmem_lessatom_sr (:918),
+//                                      segment_coo_sorted_sr (:974)
+//   test-cg/include/dataloader/dataloader.hpp SpMatCsrDescr_t (:146-168)
+//
+// Semantics kept: atomic variants add into C / dst (callers zero them first, as
+// the reference harness does), store variants overwrite; the launchers are
+// asynchronous on the default stream, except that the segment_coo family reads
+// the last index value back to size dst (the reference's segment_coo launchers
+// synchronise too, segscan.cuh:179). Differences: failures throw
+// std::runtime_error (the reference's SpMM launchers check nothing; its
+// segment_coo launchers exit()), values are read as float in every variant, and
+// scratch space for the partition / carry passes is cached in the descriptor.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../segscan.h"
+
+namespace segscan {
+
+inline void check_status(segscan_status_t st, const char* what) {
+  if (st != SEGSCAN_OK) throw std::runtime_error(std::string(what) + ": " + segscan_last_error());
+}
+
+inline void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device array with a host mirror, like util::RamArray (test-cg/include/util/ramArray.cuh:21-98).
+template <class T>
+struct DeviceArray {
+  std::vector<T> h;
+  T* d = nullptr;
+  size_t n = 0;
+  DeviceArray() = default;
+  explicit DeviceArray(size_t len) { create(len); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  DeviceArray(DeviceArray&& o) noexcept : h(std::move(o.h)), d(o.d), n(o.n) { o.d = nullptr; o.n = 0; }
+  ~DeviceArray() { if (d) (void)hipFree(d); }
+  void create(size_t len) {
+    h.assign(len, T());
+    n = len;
+    if (d) check_hip(hipFree(d), "hipFree");
+    d = nullptr;
+    if (len) check_hip(hipMalloc(&d, len * sizeof(T)), "hipMalloc");
+  }
+  void create(size_t len, const std::vector<T>& src) { create(len); std::copy(src.begin(), src.begin() + len, h.begin()); }
+  void fill_default_one() { std::fill(h.begin(), h.end(), T(1)); }
+  void fill_zero_h() { std::fill(h.begin(), h.end(), T(0)); }
+  void upload() { if (n) check_hip(hipMemcpy(d, h.data(), n * sizeof(T), hipMemcpyHostToDevice), "upload"); }
+  void download() { if (n) check_hip(hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost), "download"); }
+  void reset() { fill_zero_h(); if (n) check_hip(hipMemset(d, 0, n * sizeof(T)), "reset"); }
+};
+
+// CSR descriptor; values default to 1 as in the reference (dataloader.hpp:154-155).
+template <class Index, class DType>
+struct SpMatCsrDescr_t {
+  static_assert(sizeof(Index) == 4, "int32 indices");
+  static_assert(sizeof(DType) == 4, "fp32 values");
+  SpMatCsrDescr_t(int ncol_, std::vector<Index>& indptr, std::vector<Index>& indices) {
+    nrow = static_cast<int>(indptr.size()) - 1;
+    ncol = ncol_;
+    nnz = static_cast<int>(indices.size());
+    sp_csrptr.create(nrow + 1, indptr);
+    sp_csrind.create(nnz, indices);
+    sp_data.create(nnz);
+    sp_data.fill_default_one();
+  }
+  void upload() {
+    sp_csrptr.upload();
+    sp_csrind.upload();
+    sp_data.upload();
+  }
+  segscan_csr_t desc() const {
+    segscan_csr_t d;
+    d.nrow = nrow;
+    d.ncol = ncol;
+    d.nnz = nnz;
+    d.rowptr = reinterpret_cast<const int32_t*>(sp_csrptr.d);
+    d.colind = reinterpret_cast<const int32_t*>(sp_csrind.d);
+    d.vals = reinterpret_cast<const float*>(sp_data.d);
+    return d;
+  }
+  // Grow-only scratch for the partition / carry passes.
+  void* workspace(size_t bytes) {
+    if (bytes > ws_bytes) {
+      if (ws) check_hip(hipFree(ws), "hipFree");
+      ws = nullptr;
+      check_hip(hipMalloc(&ws, bytes), "hipMalloc workspace");
+      ws_bytes = bytes;
+    }
+    return ws;
+  }
+  ~SpMatCsrDescr_t() { if (ws) (void)hipFree(ws); }
+  int nrow = 0;
+  int ncol = 0;
+  int nnz = 0;
+  DeviceArray<Index> sp_csrptr;
+  DeviceArray<Index> sp_csrind;
+  DeviceArray<DType> sp_data;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+};
+
+namespace detail {
+
+inline segscan_knobs_t knobs(int gf, int tf, int tpb, int bn, int bd, int cf = 0, int tnz = 0, int policy = 0) {
+  segscan_knobs_t k{};
+  k.group_factor = gf == 0 ? -1 : gf;
+  k.tile_factor = tf == 0 ? -1 : tf;
+  k.thread_per_block = tpb;
+  k.block_numer = bn;
+  k.block_denom = bd;
+  k.coarsen_factor = cf;
+  k.thread_nz = tnz;
+  k.write_policy = policy;
+  return k;
+}
+
+template <class Index, class DType>
+void spmm(SpMatCsrDescr_t<Index, DType>& A, int N, const DType* B, DType* C, segscan_spmm_variant_t v,
+          const segscan_knobs_t* k, uint32_t flags) {
+  segscan_csr_t d = A.desc();
+  size_t bytes = 0;
+  check_status(segscan_spmm_workspace_size(&d, N, SEGSCAN_F32, v, k, &bytes), "spmm workspace");
+  void* ws = bytes ? A.workspace(bytes) : nullptr;
+  check_status(segscan_spmm_csr(&d, N, B, N, SEGSCAN_F32, reinterpret_cast<float*>(C), N, v, k, flags, ws, bytes,
+                                nullptr),
+               "spmm");
+}
+
+// Cached workspace for the segment_coo launchers (they take no descriptor).
+inline void* seg_workspace(size_t bytes) {
+  static thread_local void* ws = nullptr;
+  static thread_local size_t cap = 0;
+  if (bytes > cap) {
+    if (ws) check_hip(hipFree(ws), "hipFree");
+    ws = nullptr;
+    check_hip(hipMalloc(&ws, bytes), "hipMalloc workspace");
+    cap = bytes;
+  }
+  return ws;
+}
+
+template <class ValueType, class IndexType>
+void segment(const ValueType* src, const IndexType* index, int nnz, int N, ValueType* dst, segscan_seg_variant_t v,
+             const segscan_knobs_t* k) {
+  static_assert(sizeof(ValueType) == 4 && sizeof(IndexType) == 4, "float values, int32 index");
+  if (nnz <= 0 || N <= 0) return;
+  IndexType last = 0;
+  check_hip(hipMemcpy(&last, index + nnz - 1, sizeof(IndexType), hipMemcpyDeviceToHost), "read last index");
+  const int64_t dst_len = static_cast<int64_t>(last) + 1;
+  size_t bytes = 0;
+  check_status(segscan_segment_coo_workspace_size(nnz, N, dst_len, SEGSCAN_F32, v, k, &bytes), "segment workspace");
+  void* ws = bytes ? seg_workspace(bytes) : nullptr;
+  check_status(segscan_segment_coo(src, N, SEGSCAN_F32, reinterpret_cast<const int32_t*>(index), nnz, N,
+                                   reinterpret_cast<float*>(dst), N, dst_len, v, k, SEGSCAN_ACCUMULATE, ws, bytes,
+                                   nullptr),
+               "segment_coo");
+}
+
+}  // namespace detail
+
+// ---- plain SpMM (spmm.cuh) ----
+
+template <typename Index, typename DType>
+void csrspmm_parreduce_nnzbalance(SpMatCsrDescr_t<Index, DType>& A, const int N, const DType* B, DType* C) {
+  detail::spmm(A, N, B, C, SEGSCAN_SPMM_PLAIN_EBPR, nullptr, SEGSCAN_ACCUMULATE);
+}
+template <typename Index, typename DType>
+void csrspmm_parreduce_rowbalance(SpMatCsrDescr_t<Index, DType>& A, const int N, const DType* B, DType* C) {
+  detail::spmm(A, N, B, C, SEGSCAN_SPMM_PLAIN_RBPR, nullptr, 0);
+}
+template <typename Index, typename DType>
+void csrspmm_rowcaching_nnzbalance(SpMatCsrDescr_t<Index, DType>& A, const int N, const DType* B, DType* C) {
+  detail::spmm(A, N, B, C, SEGSCAN_SPMM_PLAIN_EBSR, nullptr, SEGSCAN_ACCUMULATE);
+}
+template <typename Index, typename DType>
+void csrspmm_seqreduce_rowbalance(SpMatCsrDescr_t<Index, DType>& A, const int N, const DType* B, DType* C) {
+  detail::spmm(A, N, B, C, SEGSCAN_SPMM_PLAIN_RBSR, nullptr, 0);
+}
+
+// ---- tunable SpMM (spmm_cg.cuh), same template parameter lists ----
+
+template <typename Index, typename DType, int group_factor, int thread_per_block, int tile_factor, int block_numer,
+          int block_denom>
+void csrspmm_parreduce_nnzbalance_cg(SpMatCsrDescr_t<Index, DType>& A, const int N, const DType* B, DType* C) {
+  const segscan_knobs_t k = detail::knobs(group_factor, tile_factor, thread_per_block, block_numer, block_denom);
+  detail::spmm(A, N, B, C, SEGSCAN_SPMM_EBPR, &k, SEGSCAN_ACCUMULATE);
+}
+template <typename Index, typename DType, int group_factor, int thread_per_block, int tile_factor, int block_numer,
+          int block_denom>
+void csrspmm_parreduce_rowbalance_cg(SpMatCsrDescr_t<Index, DType>& A, const int N, const DType* B, DType* C) {
+  const segscan_knobs_t k = detail::knobs(group_factor, tile_factor, thread_per_block, block_numer, block_denom);
+  detail::spmm(A, N, B, C, SEGSCAN_SPMM_RBPR, &k, SEGSCAN_ACCUMULATE);
+}
+template <typename Index, typename DType, int group_factor, int thread_per_block, int CoarsenFactor, int ThreadNz,
+          int block_numer, int block_denom>
+void csrspmm_rowcaching_nnzbalance_cg(SpMatCsrDescr_t<Index, DType>& A, const int N, const DType* B, DType* C) {
+  const segscan_knobs_t k = detail::knobs(group_factor, 0, thread_per_block, block_numer, block_denom, CoarsenFactor,
+                                          ThreadNz, SEGSCAN_WRITE_ATOMIC);
+  detail::spmm(A, N, B, C, SEGSCAN_SPMM_EBSR, &k, SEGSCAN_ACCUMULATE);
+}
+template <typename Index, typename DType, int coarsen_factor, int thread_per_block, int tile_factor, int block_numer,
+          int block_denom>
+void csrspmm_seqreduce_rowbalance_cg(SpMatCsrDescr_t<Index, DType>& A, const int N, const DType* B, DType* C) {
+  const segscan_knobs_t k =
+      detail::knobs(0, tile_factor, thread_per_block, block_numer, block_denom, coarsen_factor);
+  detail::spmm(A, N, B, C, SEGSCAN_SPMM_RBSR, &k, 0);
+}
+
+// ---- segment_coo (segscan.cuh), same template parameter lists ----
+
+template <typename ValueType, typename IndexType, int group_factor, int thread_per_block, int tile_factor,
+          int block_numer, int block_denom>
+void segment_coo(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst) {
+  const segscan_knobs_t k = detail::knobs(group_factor, tile_factor, thread_per_block, block_numer, block_denom);
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_PR, &k);
+}
+template <typename ValueType, typename IndexType, int group_factor, int thread_per_block, int tile_factor,
+          int blockDimY>
+void segment_coo_new(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst) {
+  segscan_knobs_t k = detail::knobs(group_factor, tile_factor, thread_per_block, 1, 1);
+  k.block_dim_y = blockDimY;
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_PR_NEW, &k);
+}
+template <typename ValueType, typename IndexType, int group_factor, int thread_per_block, int CoarsenFactor,
+          int ThreadNz, int block_numer, int block_denom>
+void segment_coo_sr(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst) {
+  const segscan_knobs_t k = detail::knobs(group_factor, 0, thread_per_block, block_numer, block_denom, CoarsenFactor,
+                                          ThreadNz, SEGSCAN_WRITE_ATOMIC);
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_SR, &k);
+}
+template <typename ValueType, typename IndexType, int group_factor, int thread_per_block, int CoarsenFactor,
+          int ThreadNz, int block_numer, int block_denom>
+void segment_coo_noshmem_sr(const ValueType* src, const IndexType* index, const int nnz, const int N,
+                            ValueType* dst) {
+  const segscan_knobs_t k = detail::knobs(group_factor, 0, thread_per_block, block_numer, block_denom, CoarsenFactor,
+                                          ThreadNz, SEGSCAN_WRITE_ATOMIC);
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_NOSHMEM_SR, &k);
+}
+template <typename ValueType, typename IndexType, int group_factor, int thread_per_block, int CoarsenFactor,
+          int ThreadNz, int block_numer, int block_denom>
+void segment_coo_noshmem_lessatom_sr(const ValueType* src, const IndexType* index, const int nnz, const int N,
+                                     ValueType* dst) {
+  const segscan_knobs_t k = detail::knobs(group_factor, 0, thread_per_block, block_numer, block_denom, CoarsenFactor,
+                                          ThreadNz, SEGSCAN_WRITE_LESSATOM);
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_LESSATOM_SR, &k);
+}
+template <typename ValueType, typename IndexType, int CoarsenFactor, int ThreadNz, int DtileSize, int DtileNum,
+          int BlockDimY>
+void segment_coo_sorted_sr(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst) {
+  segscan_knobs_t k = detail::knobs(0, 0, 0, 1, 1, CoarsenFactor, ThreadNz, SEGSCAN_WRITE_LESSATOM);
+  k.group_factor = 0;
+  k.tile_factor = 0;
+  k.thread_per_block = 0;
+  k.dtile_size = DtileSize;
+  k.dtile_num = DtileNum;
+  k.block_dim_y = BlockDimY;
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_SORTED_SR, &k);
+}
+
+}  // namespace segscan

@@ -76,6 +76,31 @@ void oracle_spmm_f64(int64_t M, int64_t N, const int32_t* rowptr, const int32_t*
   }
 }
 
+// Same loops with float64 operands (finite-difference gradient checks perturb the
+// inputs below fp32 resolution).
+void oracle_spmm_f64d(int64_t M, int64_t N, const int32_t* rowptr, const int32_t* colind, const double* vals,
+                      const double* B, int64_t ldb, double* C, int64_t ldc) {
+  for (int64_t i = 0; i < M; i++) {
+    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; p++) {
+      const int64_t k = colind[p];
+      const double val = vals ? vals[p] : 1.0;
+      for (int64_t j = 0; j < N; j++) C[i * ldc + j] += val * B[k * ldb + j];
+    }
+  }
+}
+
+void oracle_sddmm_f64d(int64_t M, int64_t N, const int32_t* rowptr, const int32_t* colind, const double* X,
+                       int64_t ld_x, const double* Y, int64_t ld_y, double* out) {
+  for (int64_t i = 0; i < M; i++) {
+    for (int32_t p = rowptr[i]; p < rowptr[i + 1]; p++) {
+      const int64_t k = colind[p];
+      double s = 0.0;
+      for (int64_t j = 0; j < N; j++) s += X[i * ld_x + j] * Y[k * ld_y + j];
+      out[p] = s;
+    }
+  }
+}
+
 // D[i][j] = sum_p |a_ip| |b_{k(p) j}|: the denominator of the normalized error
 // max |C - C64| / (|A| |B|).
 void oracle_spmm_abs(int64_t M, int64_t N, const int32_t* rowptr, const int32_t* colind, const float* vals,

@@ -43,6 +43,8 @@ def load():
         "oracle_transpose": (None, [c_int64, c_int64, P, P, P, P, P]),
         "oracle_sddmm_f64": (None, [c_int64, c_int64, P, P, P, c_int64, P, c_int64, P, P]),
         "oracle_generate_index": (c_int32, [c_int32, c_int32, c_int32, c_int32, c_double, c_uint32, P]),
+        "oracle_spmm_f64d": (None, [c_int64, c_int64, P, P, P, P, c_int64, P, c_int64]),
+        "oracle_sddmm_f64d": (None, [c_int64, c_int64, P, P, P, c_int64, P, c_int64, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -86,6 +88,25 @@ def spmm_f64(rowptr, colind, vals, B):
     lib.oracle_spmm_f64(M, N, _p(rp), _p(ci), _p(va), _p(B), N, _p(C), N)
     lib.oracle_spmm_abs(M, N, _p(rp), _p(ci), _p(va), _p(B), N, _p(D), N)
     return C, D
+
+
+def spmm_f64d(rowptr, colind, vals, B):
+    """float64 operands end to end (vals may be None = all ones)."""
+    rp, ci, va = _c(rowptr, np.int32), _c(colind, np.int32), _c(vals, np.float64)
+    B = _c(B, np.float64)
+    M, N = rp.size - 1, B.shape[1]
+    C = np.zeros((M, N), np.float64)
+    load().oracle_spmm_f64d(M, N, _p(rp), _p(ci), _p(va), _p(B), N, _p(C), N)
+    return C
+
+
+def sddmm_f64d(rowptr, colind, X, Y):
+    rp, ci = _c(rowptr, np.int32), _c(colind, np.int32)
+    X, Y = _c(X, np.float64), _c(Y, np.float64)
+    M, N = rp.size - 1, X.shape[1]
+    out = np.zeros(ci.size, np.float64)
+    load().oracle_sddmm_f64d(M, N, _p(rp), _p(ci), _p(X), N, _p(Y), N, _p(out))
+    return out
 
 
 def segment_coo_f32(src, index, dst_len):

@@ -93,19 +93,29 @@ def spmm_candidates(N: int, dtype: torch.dtype, stats: dict) -> list[tuple[str, 
     for c in sorted({1, 2, cf}):
         for tfac in (5, 6):
             cands.append(("rbsr", dict(coarsen_factor=c, tile_factor=tfac, thread_per_block=256)))
-    # eb_sr: a group of lanes over one dense row at 128-bit width, chunk length,
-    # workgroup size, grid size and write policy
-    for tnz in (2, 4, 8, 16, 32, 64):
-        chunk = tnz * g
-        nchunks = -(-nnz // chunk)
-        for tpb in (256, 512):
-            gpb = tpb // g
-            for waves_per_cu in (4, 8, 16):
-                blocks = max(1, min(-(-nchunks // gpb), (256 * waves_per_cu * 64) // tpb))
-                for policy in ((2, 3) if chunk >= 64 else (3, 1)):
-                    cands.append(("ebsr", dict(group_factor=gf if gf else -1, coarsen_factor=cf, thread_nz=tnz,
-                                               thread_per_block=tpb, block_numer=1,
-                                               block_denom=_blocks_to_bd(rows, blocks), write_policy=policy)))
+    # eb_sr: a group of lanes over one dense row (128-bit loads, or 64/32-bit with
+    # twice / four times the lanes), chunk length, workgroup size, grid size and
+    # write policy
+    shapes = []
+    c_, g_ = cf, g
+    while c_ >= 1 and g_ <= 64 and len(shapes) < 3:
+        shapes.append((c_, g_))
+        c_, g_ = c_ // 2, g_ * 2
+    for c_, g_ in shapes:
+        gf_ = int(math.log2(g_))
+        for tnz in (2, 4, 8, 16, 32, 64):
+            chunk = tnz * g_
+            if chunk < 32 or chunk > 2048:
+                continue
+            nchunks = -(-nnz // chunk)
+            for tpb in (256, 512):
+                gpb = tpb // g_
+                for waves_per_cu in (4, 8, 16):
+                    blocks = max(1, min(-(-nchunks // gpb), (256 * waves_per_cu * 64) // tpb))
+                    for policy in ((2, 3) if chunk >= 64 else (3, 1)):
+                        cands.append(("ebsr", dict(group_factor=gf_ if gf_ else -1, coarsen_factor=c_,
+                                                   thread_nz=tnz, thread_per_block=tpb, block_numer=1,
+                                                   block_denom=_blocks_to_bd(rows, blocks), write_policy=policy)))
     return cands
 
 
