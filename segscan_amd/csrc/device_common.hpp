@@ -262,6 +262,28 @@ __device__ __forceinline__ int group_row_find(const int32_t* __restrict__ rowptr
   return row;
 }
 
+// Wave-cooperative 64-ary search (every lane calls it with the same arguments,
+// full EXEC): the first j in [lo, hi] with a[j] > key, given a non-decreasing
+// and a[hi] > key. Each round probes 64 evenly spaced entries with one load per
+// lane and narrows the range 64-fold: 3 dependent loads for 2^18 entries
+// instead of 18 for a binary search.
+__device__ __forceinline__ int64_t wave_search_gt(const int32_t* __restrict__ a, int64_t lo, int64_t hi, int64_t key) {
+  const int lane = __lane_id();
+  while (lo < hi) {
+    const int64_t step = (hi - lo + 64) / 64;
+    // probes are non-decreasing and the last lane always probes hi
+    const int64_t j = (lane == 63) ? hi : min(lo + lane * step, hi);
+    const uint64_t gt = __ballot(a[j] > key);
+    const int f = __builtin_ctzll(gt);  // a[hi] > key, so gt != 0
+    const int64_t new_hi = (f == 63) ? hi : min(lo + f * step, hi);
+    // probe f-1 was <= key, so it was not clamped to hi
+    const int64_t new_lo = (f == 0) ? lo : lo + (f - 1) * step + 1;
+    lo = new_lo;
+    hi = new_hi;
+  }
+  return hi;
+}
+
 // Wave-cooperative lookup: every lane of the wave must call it (full EXEC).
 // Precondition: rowptr[r0] <= e for every lane and e < rowptr[M] (callers clamp
 // padding lanes to a valid position). The wave loads rowptr[r0+1 .. r0+64]
