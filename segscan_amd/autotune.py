@@ -134,12 +134,14 @@ def segment_candidates(N: int, dtype: torch.dtype, E: int) -> list[tuple[str, di
         ("segment_coo_noshmem_sr", None), ("segment_coo_noshmem_lessatom_sr", None),
         ("segment_coo_sorted_sr", None),
     ]
-    for tnz in (4, 16, 32, 64, 128):
+    # short chunks keep all groups streaming inside a narrow window of src (DRAM
+    # row locality); long chunks cut the number of chunk-boundary rows
+    for tnz in (1, 2, 4, 16, 32, 64, 128):
         chunk = tnz * g
         nchunks = -(-E // chunk)
         for tpb in (256, 512):
             gpb = tpb // g
-            for waves_per_cu in (8, 16):
+            for waves_per_cu in (4, 8, 16):
                 blocks = max(1, min(-(-nchunks // gpb), (256 * waves_per_cu * 64) // tpb))
                 cands.append(("segment_coo_noshmem_lessatom_sr",
                               dict(group_factor=gf if gf else -1, coarsen_factor=cf, thread_nz=tnz,

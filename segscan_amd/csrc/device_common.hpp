@@ -284,6 +284,28 @@ __device__ __forceinline__ int64_t wave_search_gt(const int32_t* __restrict__ a,
   return hi;
 }
 
+// Group-cooperative G-ary search (G >= 8; aligned groups, each with its own
+// group-uniform lo / hi / key): the first j in [lo, hi] with a[j] > key, given a
+// non-decreasing and a[hi] > key. Groups leave the loop independently.
+template <int G>
+__device__ __forceinline__ int64_t group_search_gt(const int32_t* __restrict__ a, int64_t lo, int64_t hi, int64_t key) {
+  static_assert(G >= 8, "use a binary search for narrower groups");
+  const int lane = __lane_id();
+  const int gbase = lane & ~(G - 1);
+  const int gl = lane - gbase;
+  while (lo < hi) {
+    const int64_t step = (hi - lo + G) / G;
+    const int64_t j = (gl == G - 1) ? hi : min(lo + gl * step, hi);
+    const uint64_t gt = (__ballot(a[j] > key) >> gbase) & group_mask<G>();
+    const int f = __builtin_ctzll(gt);  // the last lane probes hi
+    const int64_t new_hi = (f == G - 1) ? hi : min(lo + f * step, hi);
+    const int64_t new_lo = (f == 0) ? lo : lo + (f - 1) * step + 1;
+    lo = new_lo;
+    hi = new_hi;
+  }
+  return hi;
+}
+
 // Wave-cooperative lookup: every lane of the wave must call it (full EXEC).
 // Precondition: rowptr[r0] <= e for every lane and e < rowptr[M] (callers clamp
 // padding lanes to a valid position). The wave loads rowptr[r0+1 .. r0+64]

@@ -63,6 +63,23 @@ struct SrParams {
   int accumulate;
 };
 
+// SDDMM on A's pattern (sequential walk over nnz chunks).
+struct SddmmParams {
+  int M;
+  int N;
+  int64_t nnz;
+  const int32_t* rowptr;
+  const int32_t* colind;
+  const void* X;
+  int64_t ld_x;
+  const void* Y;
+  int64_t ld_y;
+  float* out;
+  int64_t chunk;
+  int64_t nchunks;
+  int64_t total_groups;
+};
+
 // Row-balanced sequential reduction (rbsr): lanes [rows][column vectors].
 struct RbsrParams {
   int M;
@@ -95,6 +112,7 @@ hipError_t launch_spmm_sr(VType vt, int cf, int g, const SrParams& p, const Laun
 hipError_t launch_spmm_rbsr(VType vt, int cf, const RbsrParams& p, const Launch& l, bool& ok);
 hipError_t launch_seg_pr(VType vt, int cf, int g, const PrParams& p, const Launch& l, bool& ok);
 hipError_t launch_seg_sr(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok);
+hipError_t launch_sddmm_sr(VType vt, int cf, int g, const SddmmParams& p, const Launch& l, bool& ok);
 
 // Helpers (misc_kernels.hip).
 hipError_t launch_fill_zero(float* C, int64_t rows, int64_t N, int64_t ldc, hipStream_t s);

@@ -211,97 +211,52 @@ hipError_t launch_transpose(const int32_t* rowptr, const int32_t* colind, const 
 }
 
 // ---- SDDMM on A's pattern: out[p] = <X[row(p), :], Y[colind[p], :]> ----------------------
-// A wave takes 64 consecutive nnz per step: every lane looks up one row (the
-// wave-cooperative rowptr window), then groups of S lanes (S*CF columns per
-// pass) take the 64 entries 64/S at a time and reduce the dot product across
-// the group.
-template <typename VT, int CF, int S>
-__global__ void __launch_bounds__(256) sddmm_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
-                                                    int M, int64_t nnz, int N, const VT* __restrict__ X, int64_t ld_x,
-                                                    const VT* __restrict__ Y, int64_t ld_y, float* __restrict__ out,
-                                                    int64_t wave_chunk) {
-  const int lane = __lane_id();
-  const int g = lane / S;
-  const int g_lane = lane - g * S;
-  constexpr int GPW = kWave / S;  // groups per wave
-  const int64_t waves_total = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
-  const int64_t wave_id = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-  for (int64_t w = wave_id; w * wave_chunk < nnz; w += waves_total) {
-    const int64_t base0 = w * wave_chunk;
-    const int64_t end0 = min(base0 + wave_chunk, nnz);
-    int r0 = row_of_nnz(rowptr, M, base0);
-    for (int64_t base = base0; base < end0; base += kWave) {
-      const int64_t e = base + lane;
-      const bool valid = e < end0;
-      const int row = wave_row_lookup(rowptr, M, r0, static_cast<int>(valid ? e : base));
-      const int col = valid ? colind[e] : 0;
-      r0 = __builtin_amdgcn_readfirstlane(row);
-      for (int st = 0; st < S; ++st) {
-        const int idx = st * GPW + g;
-        const int r = __shfl(row, idx);
-        const int k = __shfl(col, idx);
-        const bool ok = (base + idx) < end0;
-        float part = 0.f;
-        if (ok) {
-          for (int c0 = g_lane * CF; c0 < N; c0 += S * CF) {
-            const int nv = min(CF, N - c0);
-            float a[CF], b[CF];
-            load_cols_any<VT, CF>(X + static_cast<int64_t>(r) * ld_x + c0, nv, a);
-            load_cols_any<VT, CF>(Y + static_cast<int64_t>(k) * ld_y + c0, nv, b);
-#pragma unroll
-            for (int i = 0; i < CF; ++i) part += a[i] * b[i];
-          }
-        }
-        float v[1] = {part};
-        group_reduce_to_first<S, 1>(v);
-        if (ok && g_lane == 0) out[base + idx] = v[0];
-      }
-    }
-  }
-}
-
-template <typename VT>
-static hipError_t launch_sddmm_vt(const int32_t* rowptr, const int32_t* colind, int M, int64_t nnz, int N,
-                                  const void* X, int64_t ld_x, const void* Y, int64_t ld_y, float* out, int cf,
-                                  hipStream_t s) {
-  const int64_t wave_chunk = 1024;
-  const int64_t waves = (nnz + wave_chunk - 1) / wave_chunk;
-  const dim3 grid = grid_for(waves, 4, 65536);
-  const VT* x = static_cast<const VT*>(X);
-  const VT* y = static_cast<const VT*>(Y);
-  const int s_lanes_needed = (N + cf - 1) / cf;
-  // group width: smallest power of two >= columns / CF, capped at 64
-  int S = 1;
-  while (S < s_lanes_needed && S < 64) S <<= 1;
-#define SEGSCAN_SDDMM_CASE(CFv, Sv) \
-  if (cf == CFv && S == Sv) { sddmm_kernel<VT, CFv, Sv><<<grid, 256, 0, s>>>(rowptr, colind, M, nnz, N, x, ld_x, y, ld_y, out, wave_chunk); return hipGetLastError(); }
-#define SEGSCAN_SDDMM_S(CFv) \
-  SEGSCAN_SDDMM_CASE(CFv, 1) SEGSCAN_SDDMM_CASE(CFv, 2) SEGSCAN_SDDMM_CASE(CFv, 4) SEGSCAN_SDDMM_CASE(CFv, 8) \
-  SEGSCAN_SDDMM_CASE(CFv, 16) SEGSCAN_SDDMM_CASE(CFv, 32) SEGSCAN_SDDMM_CASE(CFv, 64)
-  SEGSCAN_SDDMM_S(1)
-  SEGSCAN_SDDMM_S(2)
-  SEGSCAN_SDDMM_S(4)
-#undef SEGSCAN_SDDMM_S
-#undef SEGSCAN_SDDMM_CASE
-  return hipErrorInvalidValue;
-}
+// The kernel (sddmm_sr_kernel) lives with the sequential-walk kernels; here the
+// shape: CF columns per lane at 128-bit width where N and alignment allow, a
+// group of G >= ceil(N / CF) lanes (at most 64, wider N loops), 256-nnz chunks.
 
 hipError_t launch_sddmm(VType vt, const int32_t* rowptr, const int32_t* colind, int M, int64_t nnz, int N,
                         const void* X, int64_t ld_x, const void* Y, int64_t ld_y, float* out, hipStream_t s) {
   if (nnz <= 0) return hipSuccess;
-  const size_t es = (vt == VType::F32) ? 4 : 2;
+  const int es = (vt == VType::F32) ? 4 : 2;
+  const int vec = 16 / es;
   auto aligned = [&](int cf) {
-    const size_t vb = es * cf;
-    return (reinterpret_cast<uintptr_t>(X) % vb == 0) && (reinterpret_cast<uintptr_t>(Y) % vb == 0) &&
+    const size_t vb = static_cast<size_t>(es) * cf;
+    const size_t a = vb < 16 ? vb : 16;
+    return (reinterpret_cast<uintptr_t>(X) % a == 0) && (reinterpret_cast<uintptr_t>(Y) % a == 0) &&
            (ld_x % cf == 0) && (ld_y % cf == 0);
   };
-  int cf = (N % 4 == 0 && aligned(4)) ? 4 : (N % 2 == 0 && aligned(2)) ? 2 : 1;
-  switch (vt) {
-    case VType::F32: return launch_sddmm_vt<float>(rowptr, colind, M, nnz, N, X, ld_x, Y, ld_y, out, cf, s);
-    case VType::BF16: return launch_sddmm_vt<bf16_t>(rowptr, colind, M, nnz, N, X, ld_x, Y, ld_y, out, cf, s);
-    case VType::F16: return launch_sddmm_vt<f16_t>(rowptr, colind, M, nnz, N, X, ld_x, Y, ld_y, out, cf, s);
-  }
-  return hipErrorInvalidValue;
+  int cf = 1;
+  for (int c = 2; c <= vec; c <<= 1)
+    if (N % c == 0 && aligned(c)) cf = c;
+  int g = 1;
+  while (g < 64 && g * cf < N) g <<= 1;
+  SddmmParams p{};
+  p.M = M;
+  p.N = N;
+  p.nnz = nnz;
+  p.rowptr = rowptr;
+  p.colind = colind;
+  p.X = X;
+  p.ld_x = ld_x;
+  p.Y = Y;
+  p.ld_y = ld_y;
+  p.out = out;
+  p.chunk = (g >= 256) ? g : 256;
+  p.nchunks = (nnz + p.chunk - 1) / p.chunk;
+  const int tpb = 256;
+  const int gpb = tpb / g;
+  int64_t blocks = (p.nchunks + gpb - 1) / gpb;
+  if (blocks > 4096) blocks = 4096;  // ~16 waves per CU on 256 CUs, the rest grid-strided
+  p.total_groups = blocks * gpb;
+  Launch l;
+  l.grid = dim3(static_cast<unsigned>(blocks));
+  l.block = dim3(tpb);
+  l.shmem = 0;
+  l.stream = s;
+  bool ok = true;
+  hipError_t e = launch_sddmm_sr(vt, cf, g, p, l, ok);
+  return ok ? e : hipErrorInvalidValue;
 }
 
 // ---- segment gather / counts ----

@@ -192,6 +192,104 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   }
 }
 
+// SDDMM on A's pattern with the same sequential walk: out[p] = <X[row(p), :],
+// Y[colind[p], :]>. A group of G lanes covers G*CF columns (looping over wider
+// N), walks a chunk of consecutive nnz, finds rows through its own rowptr
+// window, and per batch of U entries issues U loads of X rows (repeated rows hit
+// in cache) and U loads of Y rows before reducing each dot product across the
+// group (DPP row shifts for G <= 16).
+template <typename VT, int CF, int G>
+__global__ void __launch_bounds__(256) sddmm_sr_kernel(SddmmParams p) {
+  constexpr int U = (G < 8) ? G : 8;
+  const int L = threadIdx.x;
+  const int g = L / G;
+  const int g_lane = L - g * G;
+  const int wave_gbase = __lane_id() - g_lane;
+  const int groups_per_block = blockDim.x / G;
+  const int64_t ggid = static_cast<int64_t>(blockIdx.x) * groups_per_block + g;
+  const VT* __restrict__ X = static_cast<const VT*>(p.X);
+  const VT* __restrict__ Y = static_cast<const VT*>(p.Y);
+
+  for (int64_t c = ggid;; c += p.total_groups) {
+    if (__ballot(c < p.nchunks) == 0) break;
+    const bool in_range = c < p.nchunks;
+    const int64_t c0 = in_range ? c * p.chunk : 0;
+    const int64_t clen = in_range ? min(p.chunk, p.nnz - c0) : 0;
+    int r_hint;
+    if constexpr (G >= 8) r_hint = static_cast<int>(group_search_gt<G>(p.rowptr, 1, p.M, c0)) - 1;
+    else r_hint = row_of_nnz(p.rowptr, p.M, c0);
+
+    // metadata of the next tile: column of the lane's nnz and the rowptr window
+    int pk = 0, prp = INT32_MAX;
+    auto prefetch = [&](int64_t off_n) {
+      const bool vn = off_n + g_lane < clen;
+      pk = vn ? p.colind[c0 + off_n + g_lane] : 0;
+      if (off_n < clen) {
+        const int jj = r_hint + 1 + g_lane;
+        prp = (jj <= p.M) ? p.rowptr[jj] : INT32_MAX;
+      }
+    };
+    prefetch(0);
+    for (int64_t off = 0;; off += G) {
+      if (__ballot(off < clen) == 0) break;
+      const bool valid = off + g_lane < clen;
+      const int cnt = static_cast<int>(max<int64_t>(0, min<int64_t>(G, clen - off)));
+      const int row = group_row_find<G>(p.rowptr, p.M, r_hint, prp, static_cast<int>(c0 + off + g_lane), valid);
+      const int k = pk;
+      const int last_row = __shfl(row, wave_gbase + max(cnt, 1) - 1);
+      if (cnt > 0) r_hint = last_row;
+      prefetch(off + G);
+      for (int pp = 0; pp < cnt; pp += U) {
+        int ru[U], ku[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int src = wave_gbase + min(pp + u, cnt - 1);
+          ru[u] = __shfl(row, src);
+          ku[u] = __shfl(k, src);
+        }
+        float part[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) part[u] = 0.0f;
+        for (int cb = 0; cb < p.N; cb += G * CF) {
+          const int col0 = cb + g_lane * CF;
+          const int nv = min(CF, p.N - col0);
+          RawCols<VT, CF> xr[U], yr[U];
+          if (nv == CF) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              if (pp + u < cnt) {
+                load_raw<VT, CF>(X + static_cast<int64_t>(ru[u]) * p.ld_x + col0, xr[u]);
+                load_raw<VT, CF>(Y + static_cast<int64_t>(ku[u]) * p.ld_y + col0, yr[u]);
+              }
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            if (pp + u < cnt && nv > 0) {
+              float xa[CF], ya[CF];
+              if (nv == CF) {
+                widen<VT, CF>(xr[u], xa);
+                widen<VT, CF>(yr[u], ya);
+              } else {
+                load_cols_partial<VT, CF>(X + static_cast<int64_t>(ru[u]) * p.ld_x + col0, nv, xa);
+                load_cols_partial<VT, CF>(Y + static_cast<int64_t>(ku[u]) * p.ld_y + col0, nv, ya);
+              }
+#pragma unroll
+              for (int i = 0; i < CF; ++i) part[u] += xa[i] * ya[i];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          float v[1] = {part[u]};
+          group_reduce_to_first<G, 1>(v);
+          if (g_lane == 0 && pp + u < cnt) p.out[c0 + off + pp + u] = v[0];
+        }
+      }
+    }
+  }
+}
+
 // ---- instantiation and dispatch ----------------------------------------------------------
 
 template <typename VT, int CF, int G>
@@ -243,6 +341,50 @@ hipError_t launch_spmm_sr(VType vt, int cf, int g, const SrParams& p, const Laun
 
 hipError_t launch_seg_sr(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok) {
   return launch_sr_vt(vt, cf, g, p, l, true, ok);
+}
+
+template <typename VT, int CF, int G>
+static hipError_t launch_sddmm_t(const SddmmParams& p, const Launch& l) {
+  sddmm_sr_kernel<VT, CF, G><<<l.grid, l.block, 0, l.stream>>>(p);
+  return hipGetLastError();
+}
+
+template <typename VT, int CF>
+static hipError_t launch_sddmm_g(int g, const SddmmParams& p, const Launch& l, bool& ok) {
+  ok = true;
+  switch (g) {
+    case 1: return launch_sddmm_t<VT, CF, 1>(p, l);
+    case 2: return launch_sddmm_t<VT, CF, 2>(p, l);
+    case 4: return launch_sddmm_t<VT, CF, 4>(p, l);
+    case 8: return launch_sddmm_t<VT, CF, 8>(p, l);
+    case 16: return launch_sddmm_t<VT, CF, 16>(p, l);
+    case 32: return launch_sddmm_t<VT, CF, 32>(p, l);
+    case 64: return launch_sddmm_t<VT, CF, 64>(p, l);
+    default: ok = false; return hipSuccess;
+  }
+}
+
+template <typename VT>
+static hipError_t launch_sddmm_cf(int cf, int g, const SddmmParams& p, const Launch& l, bool& ok) {
+  switch (cf) {
+    case 1: return launch_sddmm_g<VT, 1>(g, p, l, ok);
+    case 2: return launch_sddmm_g<VT, 2>(g, p, l, ok);
+    case 4: return launch_sddmm_g<VT, 4>(g, p, l, ok);
+    case 8:
+      if constexpr (sizeof(VT) == 2) return launch_sddmm_g<VT, 8>(g, p, l, ok);
+      [[fallthrough]];
+    default: ok = false; return hipSuccess;
+  }
+}
+
+hipError_t launch_sddmm_sr(VType vt, int cf, int g, const SddmmParams& p, const Launch& l, bool& ok) {
+  switch (vt) {
+    case VType::F32: return launch_sddmm_cf<float>(cf, g, p, l, ok);
+    case VType::BF16: return launch_sddmm_cf<bf16_t>(cf, g, p, l, ok);
+    case VType::F16: return launch_sddmm_cf<f16_t>(cf, g, p, l, ok);
+  }
+  ok = false;
+  return hipSuccess;
 }
 
 }  // namespace segscan
