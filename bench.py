@@ -199,6 +199,22 @@ def main():
         fb_ms, _ = timed(fwd_bwd, 10, 3, kernel_events=False)
         fbt_ms, _ = timed(fwd_bwd_with_transpose, 5, 2, kernel_events=False)
         plain_ms = plain["plain_ebpr"]["ms"]
+        # hipSPARSE on the same inputs (the reference's cuSPARSE comparison, README §5)
+        vendor = {}
+        try:
+            from segscan_amd.vendor import VendorSpmm
+            Cv = torch.empty_like(C)
+            for alg in ("default", "csr_alg1", "csr_alg2", "csr_alg3"):
+                try:
+                    vs = VendorSpmm(A, B32, Cv, alg)
+                    ms_v, _ = timed(vs, 10, 3, kernel_events=False)
+                    vendor[alg] = round(ms_v, 5)
+                    del vs
+                except RuntimeError as exc:
+                    vendor[alg] = f"unsupported: {exc}"
+        except ImportError as exc:
+            vendor = {"error": str(exc)}
+        best_vendor = min([v for v in vendor.values() if isinstance(v, float)], default=None)
         extra = {
             "plain_f32": plain,
             "tuned_f32": {"variant": v32, "knobs": k32, "ms": round(ms_step, 5), "GBps": round(value, 1)},
@@ -206,6 +222,8 @@ def main():
                            "GBps_alg": round(b_alg(2) / (ms_bf * 1e-3) / 1e9, 1),
                            "speedup_vs_plain_ebpr_f32": round(plain_ms / ms_bf, 3),
                            "speedup_vs_tuned_f32": round(ms_step / ms_bf, 3)},
+            "hipsparse_f32_ms": vendor,
+            "hipsparse_best_over_tuned_f32": None if best_vendor is None else round(best_vendor / ms_step, 3),
             "fwd_bwd_ms": round(fb_ms, 4),
             "fwd_bwd_ms_incl_transpose": round(fbt_ms, 4),
             "gather_model_GBps_tuned_f32": round(b_gather(4) / (ms_step * 1e-3) / 1e9, 1),

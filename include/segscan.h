@@ -226,6 +226,33 @@ segscan_status_t segscan_check_index(const int32_t* index, int64_t E, int64_t ds
  * partition or fix-up pass). NULL, NULL turns the hook off. */
 void segscan_set_timing_events(void* start, void* stop);
 
+/* ---- Matrix Market input (host only) -------------------------------------------------------
+ * Replaces read_mtx_file, test-cg/include/dataloader/dataloader.hpp:22-107: coordinate
+ * files, 1-based entries made 0-based, symmetric (also skew-symmetric and hermitian)
+ * off-diagonal entries mirrored then sorted and de-duplicated; general files sorted
+ * by (row, col), duplicates kept. Without SEGSCAN_MTX_KEEP_VALUES every value is 1,
+ * as in the reference (dataloader.hpp:154-155); with it the file's values are kept
+ * (pattern files: 1, skew-symmetric mirrors: negated, complex: real part).
+ * SEGSCAN_MTX_REFERENCE_QUIRKS reproduces read_mtx_file exactly: only "symmetric"
+ * files are mirrored, and the CSR keeps only the first <declared nnz> entries of the
+ * sorted list (its loop bound, dataloader.hpp:95), which empties the last rows of a
+ * mirrored matrix. Host arrays are malloc'd; release them with segscan_host_csr_free.
+ * Errors return SEGSCAN_EINVAL with text in segscan_mtx_last_error(). */
+#define SEGSCAN_MTX_KEEP_VALUES 1u
+#define SEGSCAN_MTX_REFERENCE_QUIRKS 2u
+typedef struct segscan_host_csr {
+  int64_t nrow;
+  int64_t ncol;
+  int64_t nnz;
+  int32_t* rowptr; /* [nrow + 1], host */
+  int32_t* colind; /* [nnz], host */
+  float* vals;     /* [nnz], host */
+} segscan_host_csr_t;
+
+segscan_status_t segscan_mtx_read(const char* path, uint32_t flags, segscan_host_csr_t* out);
+void segscan_host_csr_free(segscan_host_csr_t* csr);
+const char* segscan_mtx_last_error(void);
+
 /* ---- misc -------------------------------------------------------------------------------- */
 const char* segscan_status_string(segscan_status_t status);
 const char* segscan_last_error(void);

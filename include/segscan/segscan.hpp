@@ -10,7 +10,9 @@
 //   new-cg/include/segscan/segscan.cuh segment_coo (:150), segment_coo_new (:327), segment_coo_sr (:817),
 //                                      segment_coo_noshmem_sr (:868), segment_coo_noshmem_lessatom_sr (:918),
 //                                      segment_coo_sorted_sr (:974)
-//   test-cg/include/dataloader/dataloader.hpp SpMatCsrDescr_t (:146-168)
+//   test-cg/include/dataloader/dataloader.hpp SpMatCsrDescr_t (:146-168), read_mtx_file (:22),
+//                                      compressedRow (:109), transpose (:123), DataLoader (:170),
+//                                      SingleDataLoader (:185)
 //
 // Semantics kept: atomic variants add into C / dst (callers zero them first, as
 // the reference harness does), store variants overwrite; the launchers are
@@ -28,6 +30,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../segscan.h"
@@ -108,6 +111,14 @@ struct SpMatCsrDescr_t {
     }
     return ws;
   }
+  SpMatCsrDescr_t(SpMatCsrDescr_t&& o) noexcept
+      : nrow(o.nrow), ncol(o.ncol), nnz(o.nnz), sp_csrptr(std::move(o.sp_csrptr)),
+        sp_csrind(std::move(o.sp_csrind)), sp_data(std::move(o.sp_data)), ws(o.ws), ws_bytes(o.ws_bytes) {
+    o.ws = nullptr;
+    o.ws_bytes = 0;
+  }
+  SpMatCsrDescr_t(const SpMatCsrDescr_t&) = delete;
+  SpMatCsrDescr_t& operator=(const SpMatCsrDescr_t&) = delete;
   ~SpMatCsrDescr_t() { if (ws) (void)hipFree(ws); }
   int nrow = 0;
   int ncol = 0;
@@ -275,6 +286,75 @@ void segment_coo_sorted_sr(const ValueType* src, const IndexType* index, const i
   k.dtile_num = DtileNum;
   k.block_dim_y = BlockDimY;
   detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_SORTED_SR, &k);
+}
+
+// ---- Matrix Market input (test-cg/include/dataloader/dataloader.hpp:22-192) ----------------
+// flags: SEGSCAN_MTX_REFERENCE_QUIRKS reproduces read_mtx_file's mirroring and truncation;
+// the default reads the full matrix. Values are dropped, as in the reference.
+
+// read_mtx_file (:22-107): same outputs, coo_rowind_buffer = the row of each entry.
+inline void read_mtx_file(const char* filename, int& nrow, int& ncol, int& nnz, std::vector<int>& csr_indptr_buffer,
+                          std::vector<int>& csr_indices_buffer, std::vector<int>& coo_rowind_buffer,
+                          uint32_t flags = 0) {
+  segscan_host_csr_t m;
+  if (segscan_mtx_read(filename, flags & SEGSCAN_MTX_REFERENCE_QUIRKS, &m) != SEGSCAN_OK)
+    throw std::runtime_error(std::string("read_mtx_file: ") + segscan_mtx_last_error());
+  nrow = static_cast<int>(m.nrow);
+  ncol = static_cast<int>(m.ncol);
+  nnz = static_cast<int>(m.nnz);
+  csr_indptr_buffer.assign(m.rowptr, m.rowptr + m.nrow + 1);
+  csr_indices_buffer.assign(m.colind, m.colind + m.nnz);
+  coo_rowind_buffer.resize(static_cast<size_t>(m.nnz));
+  for (int r = 0; r < nrow; ++r)
+    std::fill(coo_rowind_buffer.begin() + m.rowptr[r], coo_rowind_buffer.begin() + m.rowptr[r + 1], r);
+  segscan_host_csr_free(&m);
+}
+
+// compressedRow (:109-121): row pointer of a row-sorted row-index list.
+template <class Index>
+void compressedRow(Index nrow, const std::vector<Index>& rowind, std::vector<Index>& rowptr) {
+  rowptr.assign(static_cast<size_t>(nrow) + 1, 0);
+  for (Index r : rowind) ++rowptr[static_cast<size_t>(r) + 1];
+  for (Index r = 0; r < nrow; ++r) rowptr[r + 1] += rowptr[r];
+}
+
+// transpose (:123-144): stable counting sort of the entries by column; row_t gets the
+// column ids (sorted), col_t the row ids (ascending within a column).
+template <class Index>
+void transpose(Index ncol, const std::vector<Index>& row, const std::vector<Index>& col, std::vector<Index>& row_t,
+               std::vector<Index>& col_t) {
+  std::vector<Index> next(static_cast<size_t>(ncol) + 1, 0);
+  for (Index c : col) ++next[static_cast<size_t>(c) + 1];
+  for (Index c = 0; c < ncol; ++c) next[c + 1] += next[c];
+  row_t.resize(col.size());
+  col_t.resize(col.size());
+  for (size_t e = 0; e < col.size(); ++e) {
+    const Index q = next[col[e]]++;
+    row_t[q] = col[e];
+    col_t[q] = row[e];
+  }
+}
+
+// SingleDataLoader (:185-192).
+template <class Index, class DType>
+SpMatCsrDescr_t<Index, DType> SingleDataLoader(const char* filename, uint32_t flags = 0) {
+  int nrow, ncol, nnz;
+  std::vector<Index> indptr, indices, coorow;
+  read_mtx_file(filename, nrow, ncol, nnz, indptr, indices, coorow, flags);
+  return SpMatCsrDescr_t<Index, DType>(ncol, indptr, indices);
+}
+
+// DataLoader (:170-183): the matrix and its transpose, both built on the host.
+template <class Index, class DType>
+std::tuple<SpMatCsrDescr_t<Index, DType>, SpMatCsrDescr_t<Index, DType>> DataLoader(const char* filename,
+                                                                                   uint32_t flags = 0) {
+  int nrow, ncol, nnz;
+  std::vector<Index> indptr, indices, coorow, t_indptr, t_coorow, t_indices;
+  read_mtx_file(filename, nrow, ncol, nnz, indptr, indices, coorow, flags);
+  transpose<Index>(ncol, coorow, indices, t_coorow, t_indices);
+  compressedRow<Index>(ncol, t_coorow, t_indptr);
+  return std::make_tuple(SpMatCsrDescr_t<Index, DType>(ncol, indptr, indices),
+                         SpMatCsrDescr_t<Index, DType>(nrow, t_indptr, t_indices));
 }
 
 }  // namespace segscan

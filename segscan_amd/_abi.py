@@ -25,6 +25,8 @@ SEG_PR, SEG_PR_NEW, SEG_SR, SEG_NOSHMEM_SR, SEG_LESSATOM_SR, SEG_SORTED_SR, SEG_
 # segscan_write_policy_t
 WRITE_DEFAULT, WRITE_ATOMIC, WRITE_OWNER, WRITE_LESSATOM = 0, 1, 2, 3
 ACCUMULATE = 1
+# segscan_mtx_read flags
+MTX_KEEP_VALUES, MTX_REFERENCE_QUIRKS = 1, 2
 
 SPMM_VARIANTS = {
     "ebpr": SPMM_EBPR, "rbpr": SPMM_RBPR, "ebsr": SPMM_EBSR, "rbsr": SPMM_RBSR,
@@ -42,6 +44,13 @@ class CsrDesc(ctypes.Structure):
     _fields_ = [
         ("nrow", c_int64), ("ncol", c_int64), ("nnz", c_int64),
         ("rowptr", c_void_p), ("colind", c_void_p), ("vals", c_void_p),
+    ]
+
+
+class HostCsrDesc(ctypes.Structure):
+    _fields_ = [
+        ("nrow", c_int64), ("ncol", c_int64), ("nnz", c_int64),
+        ("rowptr", POINTER(c_int32)), ("colind", POINTER(c_int32)), ("vals", POINTER(ctypes.c_float)),
     ]
 
 
@@ -110,6 +119,9 @@ _SIGNATURES = {
     "segscan_check_csr": (c_int32, [POINTER(CsrDesc), c_void_p, c_void_p]),
     "segscan_check_index": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
     "segscan_set_timing_events": (None, [c_void_p, c_void_p]),
+    "segscan_mtx_read": (c_int32, [c_char_p, c_uint32, c_void_p]),
+    "segscan_host_csr_free": (None, [c_void_p]),
+    "segscan_mtx_last_error": (c_char_p, []),
 }
 
 EXPORTED = tuple(_SIGNATURES)

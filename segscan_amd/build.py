@@ -25,6 +25,7 @@ SOURCES = [
     "rbsr_kernels.hip",
     "misc_kernels.hip",
     "segscan_abi.cpp",
+    "mtx_loader.cpp",
 ]
 
 ARCH = os.environ.get("SEGSCAN_OFFLOAD_ARCH", "gfx950")
@@ -72,7 +73,25 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
+    build_vendor(verbose)
     return LIB
+
+
+VENDOR_LIB = PKG / "libsegscan_vendor.so"
+
+
+def build_vendor(verbose: bool = True) -> Path:
+    """The hipSPARSE comparator (timing only), in a library of its own."""
+    src = CSRC / "vendor_hipsparse.cpp"
+    tmp = VENDOR_LIB.with_suffix(".so.tmp")
+    cmd = [hipcc(), *CXXFLAGS, "-shared", str(src), "-lhipsparse", "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src.name}:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, VENDOR_LIB)
+    if verbose:
+        print(f"built {VENDOR_LIB}", file=sys.stderr)
+    return VENDOR_LIB
 
 
 if __name__ == "__main__":

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
@@ -77,13 +78,10 @@ static void check_spmm(const char* name, SpMatCsrDescr_t<Index, DType>& H, int N
   report(name, pass);
 }
 
-static void spmm_suite(int M, int K, int N, bool integer_values, uint64_t seed) {
-  std::vector<Index> indptr, indices;
-  make_csr(M, K, seed, indptr, indices);
-  SpMatCsrDescr_t<Index, DType> H(K, indptr, indices);
+static void spmm_checks(SpMatCsrDescr_t<Index, DType>& H, int N, bool integer_values, uint64_t seed) {
+  const int M = H.nrow, K = H.ncol;
   std::mt19937_64 rng(seed + 1);
   std::uniform_int_distribution<int> small(-2, 2);
-  std::uniform_real_distribution<float> unif(-1.f, 1.f);
   if (integer_values) {
     for (auto& v : H.sp_data.h) v = static_cast<float>(small(rng));
   }  // else: values 1 as the reference's SpMatCsrDescr_t
@@ -96,7 +94,8 @@ static void spmm_suite(int M, int K, int N, bool integer_values, uint64_t seed) 
   H.upload();
   in_feature.upload();
   std::vector<float> out_ref(static_cast<size_t>(M) * N, 0.f);
-  oracle_spmm_f32(M, N, indptr.data(), indices.data(), H.sp_data.h.data(), in_feature.h.data(), N, out_ref.data(), N);
+  oracle_spmm_f32(M, N, H.sp_csrptr.h.data(), H.sp_csrind.h.data(), H.sp_data.h.data(), in_feature.h.data(), N,
+                  out_ref.data(), N);
   DType* B = in_feature.d;
   DType* C = out_feature.d;
   char tag[96];
@@ -126,6 +125,40 @@ static void spmm_suite(int M, int K, int N, bool integer_values, uint64_t seed) 
       [&] { segscan::csrspmm_seqreduce_rowbalance_cg<Index, DType, 4, 256, 5, 1, 1>(H, N, B, C); });
   run("csrspmm_seqreduce_rowbalance_cg<2,256,0,1,32>",
       [&] { segscan::csrspmm_seqreduce_rowbalance_cg<Index, DType, 2, 256, 0, 1, 32>(H, N, B, C); });
+}
+
+static void spmm_suite(int M, int K, int N, bool integer_values, uint64_t seed) {
+  std::vector<Index> indptr, indices;
+  make_csr(M, K, seed, indptr, indices);
+  SpMatCsrDescr_t<Index, DType> H(K, indptr, indices);
+  spmm_checks(H, N, integer_values, seed);
+}
+
+// The reference's check mode on a Matrix Market file (test.cu:23-82 via SingleDataLoader,
+// values and B all ones), plus DataLoader's host transpose against the device transpose.
+static void mtx_suite(const char* path, int N, uint32_t flags) {
+  std::printf("-- %s%s\n", path, (flags & SEGSCAN_MTX_REFERENCE_QUIRKS) ? " (reference quirks)" : "");
+  auto H = segscan::SingleDataLoader<Index, DType>(path, flags);
+  spmm_checks(H, N, false, 7);
+  auto both = segscan::DataLoader<Index, DType>(path, flags);
+  auto& A = std::get<0>(both);
+  auto& At = std::get<1>(both);
+  A.upload();
+  segscan_csr_t d = A.desc();
+  size_t ws_bytes = 0;
+  segscan::check_status(segscan_csr_transpose_workspace_size(&d, &ws_bytes), "transpose workspace");
+  DeviceArray<int> colptr(static_cast<size_t>(A.ncol) + 1), rowind(static_cast<size_t>(A.nnz) + 1),
+      perm(static_cast<size_t>(A.nnz) + 1);
+  void* ws = A.workspace(ws_bytes);
+  segscan::check_status(segscan_csr_transpose(&d, colptr.d, rowind.d, perm.d, nullptr, ws, ws_bytes, nullptr),
+                        "transpose");
+  segscan::check_hip(hipDeviceSynchronize(), "sync");
+  colptr.download();
+  rowind.download();
+  bool pass = At.nrow == A.ncol && At.ncol == A.nrow && At.nnz == A.nnz &&
+              std::equal(At.sp_csrptr.h.begin(), At.sp_csrptr.h.end(), colptr.h.begin()) &&
+              std::equal(At.sp_csrind.h.begin(), At.sp_csrind.h.end(), rowind.h.begin());
+  report("DataLoader transpose == segscan_csr_transpose", pass);
 }
 
 template <class F>
@@ -182,7 +215,15 @@ static void segment_suite() {
   });
 }
 
-int main() {
+// Usage: spmm_check                      synthetic suites
+//        spmm_check <m.mtx> <N> [quirks]  the check mode on a Matrix Market file
+static int run_main(int argc, char** argv) {
+  if (argc >= 3) {
+    mtx_suite(argv[1], std::atoi(argv[2]), (argc >= 4 && std::strcmp(argv[3], "quirks") == 0)
+                                               ? SEGSCAN_MTX_REFERENCE_QUIRKS : 0u);
+    std::printf("%d launches, %d failed\n", g_launches, g_failures);
+    return g_failures ? 1 : 0;
+  }
   spmm_suite(3000, 2500, 64, true, 1);
   spmm_suite(3000, 2500, 33, true, 2);
   spmm_suite(2000, 2000, 16, false, 3);
@@ -190,4 +231,14 @@ int main() {
   segment_suite();
   std::printf("%d launches, %d failed\n", g_launches, g_failures);
   return g_failures ? 1 : 0;
+}
+
+int main(int argc, char** argv) {
+  try {
+    return run_main(argc, argv);
+  } catch (const std::exception& e) {
+    std::fflush(stdout);
+    std::fprintf(stderr, "spmm_check: %s\n", e.what());
+    return 2;
+  }
 }
