@@ -105,11 +105,11 @@ def main():
     tuned = {}
     dump = {}
     for dtype, B in ((torch.float32, B32), (torch.bfloat16, Bbf)):
-        key = autotune.table_key("spmm", dtype, N, stats)
-        hit = autotune.lookup(table, key)
-        if hit is not None:
-            tuned[dtype] = hit
-            log(f"tuning table hit {key}: {hit}")
+        key = autotune.matrix_key("spmm", dtype, N, stats, m.ncol)
+        entry = table.get(key)  # this matrix's own tuning result; bucket entries are for other matrices
+        if entry is not None:
+            tuned[dtype] = (entry["variant"], autotune.resolve_knobs(entry.get("knobs"), stats))
+            log(f"tuning table hit {key}: {tuned[dtype]}")
             continue
         cands = autotune.spmm_candidates(N, dtype, stats)
         t1 = time.time()

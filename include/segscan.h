@@ -135,6 +135,8 @@ typedef struct segscan_csr {
   const int32_t* rowptr; /* [nrow + 1], device */
   const int32_t* colind; /* [nnz], device */
   const float* vals;     /* [nnz], device, or NULL */
+  int64_t max_row_nnz;   /* longest row, or 0 = unknown; guides SEGSCAN_SPMM_AUTO only
+                            (0 <= max_row_nnz <= nnz). Zero-initialise the struct. */
 } segscan_csr_t;
 
 /* Flags. Without SEGSCAN_ACCUMULATE the call computes C = A.B (resp. dst =

@@ -85,6 +85,7 @@ struct SpMatCsrDescr_t {
     sp_csrind.create(nnz, indices);
     sp_data.create(nnz);
     sp_data.fill_default_one();
+    for (int r = 0; r < nrow; ++r) max_row = std::max<int>(max_row, indptr[r + 1] - indptr[r]);
   }
   void upload() {
     sp_csrptr.upload();
@@ -99,6 +100,7 @@ struct SpMatCsrDescr_t {
     d.rowptr = reinterpret_cast<const int32_t*>(sp_csrptr.d);
     d.colind = reinterpret_cast<const int32_t*>(sp_csrind.d);
     d.vals = reinterpret_cast<const float*>(sp_data.d);
+    d.max_row_nnz = max_row;
     return d;
   }
   // Grow-only scratch for the partition / carry passes.
@@ -112,7 +114,7 @@ struct SpMatCsrDescr_t {
     return ws;
   }
   SpMatCsrDescr_t(SpMatCsrDescr_t&& o) noexcept
-      : nrow(o.nrow), ncol(o.ncol), nnz(o.nnz), sp_csrptr(std::move(o.sp_csrptr)),
+      : nrow(o.nrow), ncol(o.ncol), nnz(o.nnz), max_row(o.max_row), sp_csrptr(std::move(o.sp_csrptr)),
         sp_csrind(std::move(o.sp_csrind)), sp_data(std::move(o.sp_data)), ws(o.ws), ws_bytes(o.ws_bytes) {
     o.ws = nullptr;
     o.ws_bytes = 0;
@@ -123,6 +125,7 @@ struct SpMatCsrDescr_t {
   int nrow = 0;
   int ncol = 0;
   int nnz = 0;
+  int max_row = 0;  // longest row; guides SEGSCAN_SPMM_AUTO
   DeviceArray<Index> sp_csrptr;
   DeviceArray<Index> sp_csrind;
   DeviceArray<DType> sp_data;

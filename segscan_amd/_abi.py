@@ -44,6 +44,7 @@ class CsrDesc(ctypes.Structure):
     _fields_ = [
         ("nrow", c_int64), ("ncol", c_int64), ("nnz", c_int64),
         ("rowptr", c_void_p), ("colind", c_void_p), ("vals", c_void_p),
+        ("max_row_nnz", c_int64),
     ]
 
 

@@ -21,16 +21,16 @@ class SpmmPattern:
 
     def __init__(self, A: ops.CsrMatrix, variant="auto", knobs=None, variant_t="auto", knobs_t=None):
         self.A = A
-        self.T = ops.csr_transpose(A)
+        T = ops.csr_transpose(A)
+        self.T = ops.CscMatrix(T.t.measured(), T.perm)  # longest column guides the "auto" plan of A^T
         self.variant, self.knobs = variant, knobs
         self.variant_t, self.knobs_t = variant_t, knobs_t
 
     def with_vals(self, vals: torch.Tensor) -> ops.CsrMatrix:
-        return ops.CsrMatrix(self.A.nrow, self.A.ncol, self.A.rowptr, self.A.colind, vals)
+        return self.A.with_vals(vals)
 
     def transposed_with_vals(self, vals: torch.Tensor) -> ops.CsrMatrix:
-        t = self.T.t
-        return ops.CsrMatrix(t.nrow, t.ncol, t.rowptr, t.colind, vals.index_select(0, self.T.perm.long()))
+        return self.T.t.with_vals(vals.index_select(0, self.T.perm.long()))
 
 
 class SpmmFunction(torch.autograd.Function):

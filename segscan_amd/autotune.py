@@ -1,6 +1,5 @@
-"""Autotuner: picks SpMM / segment_coo variant and knobs per (row-length skew,
-N, dtype) bucket by timing candidates on the device, and keeps the winners in a
-JSON table.
+"""Autotuner: picks SpMM / segment_coo variant and knobs per (N, row-length
+skew, size) bucket, and per matrix, and keeps the winners in a JSON table.
 
 It replaces the reference's brute-force sweeps (test-cg/src/params.md: rb-pr
 :1-100, eb_sr :102-177, rb_sr :179-235, eb_pr :237-438, driven under Nsight by
@@ -9,10 +8,18 @@ test-cg/src/test.py:42-59) and the static-vs-dynamic analysis
 reference's knob points mapped to wave64 plus the gfx950-specific ones (owner
 write policy, long chunks, 16-lane groups over a 256-byte row).
 
-Table key: "<arch>|<kind>|<dtype>|N<n_bucket>|skew<skew_bucket>" where
-n_bucket = N rounded up to a power of two and skew_bucket = round(log2(max
-row length / mean row length)). Timing uses HIP events on torch's current
-stream around each candidate (median of `reps` after one warm-up).
+Two kinds of table entries:
+  bucket  "<arch>|<kind>|<dtype>|N<n>|skew<c>|<size>": fitted from a sweep
+          (``fit_table``): the configuration with the lowest geometric mean of
+          time / per-matrix best over the sweep's matrices in the bucket;
+  matrix  bucket key + "|m<rows>x<cols>:<nnz>:<max row>": the winner of a
+          per-matrix tuning run (``tune_spmm``), preferred when present.
+n = N rounded up to a power of two; c = 0 for even rows (longest row <= 4 x
+mean + 32, the rule the C planner uses), else 1 + floor(log2(max / mean) / 4)
+capped at 3; size = "s" below 2^20 nonzeros, else "l". Knobs may hold
+"waves_per_cu" instead of a block factor; ``resolve_knobs`` turns it into a
+grid for the matrix at hand. Timing uses HIP events on torch's current stream
+with the host-side issue cost hidden (``time_call``).
 """
 from __future__ import annotations
 
@@ -44,19 +51,50 @@ def row_stats(rowptr: np.ndarray) -> dict:
     }
 
 
-def skew_bucket(stats: dict) -> int:
-    if stats["mean"] <= 0:
+def skew_class(stats: dict) -> int:
+    mean, mx = stats["mean"], stats["max"]
+    if mean <= 0 or mx <= 4 * mean + 32:
         return 0
-    return int(round(math.log2(max(1.0, stats["max"] / stats["mean"]))))
+    return 1 + min(2, int(math.log2(mx / mean) // 4))
+
+
+def size_class(stats: dict) -> str:
+    return "s" if stats["nnz"] < (1 << 20) else "l"
 
 
 def n_bucket(N: int) -> int:
     return 1 << max(0, int(math.ceil(math.log2(max(1, N)))))
 
 
-def table_key(kind: str, dtype: torch.dtype, N: int, stats: dict, arch: str = "gfx950") -> str:
-    dt = {torch.float32: "f32", torch.bfloat16: "bf16", torch.float16: "f16"}[dtype]
-    return f"{arch}|{kind}|{dt}|N{n_bucket(N)}|skew{skew_bucket(stats)}"
+_DT = {torch.float32: "f32", torch.bfloat16: "bf16", torch.float16: "f16"}
+
+
+def table_key(kind: str, dtype, N: int, stats: dict, arch: str = "gfx950") -> str:
+    dt = _DT.get(dtype, dtype)
+    return f"{arch}|{kind}|{dt}|N{n_bucket(N)}|skew{skew_class(stats)}|{size_class(stats)}"
+
+
+def matrix_key(kind: str, dtype, N: int, stats: dict, ncol: int, arch: str = "gfx950") -> str:
+    return table_key(kind, dtype, N, stats, arch) + f"|m{stats['rows']}x{ncol}:{stats['nnz']}:{stats['max']}"
+
+
+def resolve_knobs(knobs: dict | None, stats: dict) -> dict | None:
+    """Concrete knobs for one matrix: "waves_per_cu" becomes a grid of
+    min(chunks / groups per workgroup, 256 CUs x waves x 64 lanes / workgroup)
+    workgroups, expressed as the block factor 1 / (rows / grid)."""
+    if not knobs or "waves_per_cu" not in knobs:
+        return knobs
+    p = dict(knobs)
+    waves = p.pop("waves_per_cu")
+    gf = p.get("group_factor", 0)
+    g = 1 << gf if gf > 0 else 1
+    chunk = p["thread_nz"] * g
+    nchunks = -(-max(1, stats["nnz"]) // chunk)
+    gpb = max(1, p["thread_per_block"] // g)
+    blocks = max(1, min(-(-nchunks // gpb), (256 * waves * 64) // p["thread_per_block"]))
+    p["block_numer"] = 1
+    p["block_denom"] = max(1, stats["rows"] // blocks)
+    return p
 
 
 # ---- search spaces -------------------------------------------------------------------------
@@ -152,13 +190,38 @@ def segment_candidates(N: int, dtype: torch.dtype, E: int) -> list[tuple[str, di
 
 # ---- timing ----------------------------------------------------------------------------------
 
-def time_call(fn, reps: int = 10) -> float:
-    """Median milliseconds of fn() over reps (after one warm-up), HIP events on the current stream."""
+_SLEEP_CYCLES_PER_MS: dict[int, float] = {}
+
+
+def _sleep_cycles_per_ms() -> float:
+    """Rate of torch.cuda._sleep (a spin kernel) on the current device, measured once."""
+    dev = torch.cuda.current_device()
+    if dev not in _SLEEP_CYCLES_PER_MS:
+        cycles = 1_000_000
+        torch.cuda._sleep(cycles)
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record()
+        torch.cuda._sleep(cycles)
+        e.record()
+        e.synchronize()
+        _SLEEP_CYCLES_PER_MS[dev] = cycles / max(1e-3, s.elapsed_time(e))
+    return _SLEEP_CYCLES_PER_MS[dev]
+
+
+def time_call(fn, reps: int = 10, hide_host: bool = True) -> float:
+    """Median milliseconds of fn() over reps (after one warm-up), HIP events on the
+    current stream. With hide_host, a 0.3 ms spin kernel is queued before the start
+    event, so the host-side cost of issuing fn() overlaps it and the interval holds
+    only the device work (kernels and the gaps between them)."""
     fn()
+    cycles = int(0.3 * _sleep_cycles_per_ms()) if hide_host else 0
     times = []
     for _ in range(reps):
         s = torch.cuda.Event(enable_timing=True)
         e = torch.cuda.Event(enable_timing=True)
+        if cycles:
+            torch.cuda._sleep(cycles)
         s.record()
         fn()
         e.record()
@@ -212,8 +275,68 @@ def save_table(table: dict, path: Path | str = DEFAULT_TABLE) -> None:
     Path(path).write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
 
 
-def lookup(table: dict, key: str) -> tuple[str, dict | None] | None:
-    entry = table.get(key)
-    if entry is None:
-        return None
-    return entry["variant"], entry.get("knobs")
+def lookup(table: dict, kind: str, dtype, N: int, stats: dict, ncol: int | None = None):
+    """(variant, concrete knobs, key) from the matrix entry if present, else the
+    bucket entry, else None."""
+    keys = ([matrix_key(kind, dtype, N, stats, ncol)] if ncol is not None else []) + [table_key(kind, dtype, N, stats)]
+    for key in keys:
+        entry = table.get(key)
+        if entry is not None:
+            return entry["variant"], resolve_knobs(entry.get("knobs"), stats), key
+    return None
+
+
+def fit_table(rows: list[dict], kind: str = "spmm", exclude: str | None = None) -> dict:
+    """Bucket entries from sweep rows (segscan_amd.sweep run output): per (dtype,
+    N bucket, skew class, size class) the action with the lowest geometric mean of
+    time / per-matrix best among the actions timed on every matrix of the bucket.
+    ``exclude`` leaves one matrix out (for leave-one-out evaluation)."""
+    families = ("ebpr", "rbpr", "ebsr", "rbsr")
+    groups: dict = {}
+    spec: dict = {}
+    for r in rows:
+        if r["dataset"] == exclude or r.get("family") not in families:
+            continue
+        t = r.get("ms")
+        if t is None or not np.isfinite(t) or t <= 0:
+            continue
+        stats = {"rows": r["rows"], "nnz": r["nnz"], "max": r["max_row"],
+                 "mean": r["nnz"] / max(1, r["rows"])}
+        key = table_key(kind, r["dtype"], r["N"], stats)
+        groups.setdefault(key, {}).setdefault(r["dataset"], {})[r["action"]] = float(t)
+        spec[r["action"]] = r
+    table = {}
+    for key, per_ds in groups.items():
+        common = set.intersection(*(set(v) for v in per_ds.values()))
+        if not common:
+            continue
+        best = {d: min(v.values()) for d, v in per_ds.items()}
+        score = {a: float(np.exp(np.mean([np.log(per_ds[d][a] / best[d]) for d in per_ds]))) for a in common}
+        a = min(sorted(score), key=score.get)
+        r = spec[a]
+        res = json.loads(r["resolved"]) if r.get("resolved") else {}
+        table[key] = {"variant": res.get("variant", r.get("variant")), "knobs": r.get("params") or res.get("knobs"),
+                      "action": a, "gmean_vs_best": round(score[a], 4), "matrices": sorted(per_ds)}
+    return table
+
+
+def _main(argv=None) -> int:
+    import argparse
+    ap = argparse.ArgumentParser(prog="python -m segscan_amd.autotune")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    f = sub.add_parser("fit", help="write bucket entries fitted from sweep output into the table")
+    f.add_argument("sweep", nargs="+", help="segscan_amd.sweep run output (JSON lines)")
+    f.add_argument("--table", default=str(DEFAULT_TABLE))
+    a = ap.parse_args(argv)
+    rows = [json.loads(x) for p in a.sweep for x in Path(p).read_text().splitlines() if x.strip()]
+    table = {k: v for k, v in load_table(a.table).items() if "|m" in k}  # keep per-matrix entries
+    fitted = fit_table(rows)
+    table.update(fitted)
+    save_table(table, a.table)
+    for k, v in sorted(fitted.items()):
+        print(f"{k}: {v['action']} (gmean {v['gmean_vs_best']}, {len(v['matrices'])} matrices)")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())

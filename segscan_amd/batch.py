@@ -54,8 +54,7 @@ class BatchedGcnStep:
         self.variant_t, self.knobs_t = variant_t, knobs_t
         self.seg_variant, self.seg_knobs = seg_variant, seg_knobs
         T = ops.csr_transpose(A)
-        self.At = ops.CsrMatrix(T.t.nrow, T.t.ncol, T.t.rowptr, T.t.colind,
-                                None if A.vals is None else A.vals.index_select(0, T.perm.long()))
+        self.At = T.t.measured().with_vals(None if A.vals is None else A.vals.index_select(0, T.perm.long()))
         self.H = torch.empty((A.nrow, N), dtype=torch.float32, device=dev)
         self.P = torch.empty((batch.n_graphs, N), dtype=torch.float32, device=dev)
         self.dHt = torch.empty((A.nrow, N), dtype=torch.float32, device=dev)

@@ -175,26 +175,56 @@ struct SpmmPlan {
 
 // Reference defaults: README.md:168-210 best static tuples for the cg variants;
 // spmm.cuh plain launchers for the plain ones.
-segscan_status_t plan_spmm(int64_t M, int64_t nnz, int64_t N, segscan_dtype_t bt, int variant, const Knobs& user,
-                           int cf_cap, SpmmPlan& pl) {
+segscan_status_t plan_spmm(int64_t M, int64_t nnz, int64_t max_row, int64_t N, segscan_dtype_t bt, int variant,
+                           const Knobs& user, int cf_cap, SpmmPlan& pl) {
   if (variant < 0 || variant > SEGSCAN_SPMM_AUTO) return fail(SEGSCAN_EINVAL, "unknown SpMM variant %d", variant);
   const int vec = vec_elems(bt);
   Knobs k = user;
   int v = variant;
   if (v == SEGSCAN_SPMM_AUTO) {
-    // Built-in choice for gfx950: element-balanced sequential reduction, a group
-    // of lanes per dense row at 128-bit width, owner stores, 256-nnz chunks.
-    v = SEGSCAN_SPMM_EBSR;
+    // Built-in choice for gfx950, fitted to a sweep over uniform, power-law,
+    // banded, rectangular and dense-row matrices at N = 4..128 (fp32, bf16):
+    //  * rows known to be even (longest row <= 4 x mean + 32): one group of
+    //    lanes per row — row-balanced parallel reduction with a group of about
+    //    half a row for N <= 8 (and N <= 16 with rows of >= 24 nonzeros), else
+    //    one 256-byte slice of the dense row per group of lanes (row-balanced
+    //    sequential reduction);
+    //  * otherwise (skewed, or the longest row unknown): element-balanced
+    //    sequential reduction, <= 4 columns per lane and a group of >= 16
+    //    lanes (or N) over a dense row, owner stores, chunks of >= 64 nonzeros
+    //    sized so that ~16 waves per CU each take about one chunk
+    //    (thread_nz = nnz / 2^18, within [64 / group, 32]).
+    const double mean = M > 0 ? static_cast<double>(nnz) / static_cast<double>(M) : 0.0;
+    const bool even = max_row > 0 && static_cast<double>(max_row) <= 4.0 * mean + 32.0;
     int cf = std::min(vec, divisor_cf(N, vec));
-    int lanes = static_cast<int>(std::min<int64_t>(64, ceil_div(N, cf)));
-    int g = 1;
-    while (g < lanes) g <<= 1;
-    int gf = 0;
-    while ((1 << gf) < g) ++gf;
-    k = mk(gf == 0 ? -1 : gf, 5, 256, 1, 1, cf, std::max(1, 256 / g), SEGSCAN_WRITE_OWNER);
-    // enough workgroups for ~8 waves per CU on 256 CUs, the rest grid-strided
-    k.bn = 1;
-    k.bd = std::max<int>(1, static_cast<int>(M / 2048));
+    if (even && (N <= 8 || (N <= 16 && mean >= 24.0))) {
+      v = SEGSCAN_SPMM_RBPR;
+      int gf = 2;
+      while (gf < 4 && static_cast<double>(int64_t{1} << (gf + 1)) <= mean / 2.0) ++gf;
+      k = mk(gf, gf, 256, 1, 2, 0, 1, 0);
+    } else if (even) {
+      v = SEGSCAN_SPMM_RBSR;
+      k = mk(-1, 6, 256, 1, 1, cf, 1, 0);
+    } else {
+      v = SEGSCAN_SPMM_EBSR;
+      cf = std::min(cf, 4);
+      while (cf > 1 && N / cf < std::min<int64_t>(16, N)) cf >>= 1;
+      int lanes = static_cast<int>(std::min<int64_t>(64, ceil_div(N, cf)));
+      int g = 1;
+      while (g < lanes) g <<= 1;
+      int gf = 0;
+      while ((1 << gf) < g) ++gf;
+      int tnz = std::max(1, 64 / g);
+      while (tnz < 32 && (static_cast<int64_t>(tnz) << 19) <= nnz) tnz <<= 1;
+      const int64_t chunk = static_cast<int64_t>(tnz) * g;
+      const int64_t gpb = 256 / g;
+      const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(ceil_div(nnz, chunk), gpb),
+                                                                     256 * 16 * 64 / 256));
+      k = mk(gf == 0 ? -1 : gf, 5, 256, 1, 1, cf, tnz, chunk >= 64 ? SEGSCAN_WRITE_OWNER : SEGSCAN_WRITE_LESSATOM);
+      // workers(M) = M * bn / bd = blocks
+      k.bn = static_cast<int>(blocks);
+      k.bd = static_cast<int>(std::max<int64_t>(1, M));
+    }
   }
   pl.variant = v;
   switch (v) {
@@ -355,6 +385,8 @@ segscan_status_t validate_csr(const segscan_csr_t* A, bool need_pointers) {
   if (A->nnz > kMaxI32) return fail(SEGSCAN_EINVAL, "nnz %lld exceeds int32 indexing", (long long)A->nnz);
   if (A->nrow == 0 && A->nnz > 0) return fail(SEGSCAN_EINVAL, "nnz > 0 with no rows");
   if (A->ncol == 0 && A->nnz > 0) return fail(SEGSCAN_EINVAL, "nnz > 0 with no columns");
+  if (A->max_row_nnz < 0 || A->max_row_nnz > A->nnz)
+    return fail(SEGSCAN_EINVAL, "max_row_nnz %lld outside [0, nnz]", (long long)A->max_row_nnz);
   if (need_pointers) {
     if (A->nrow > 0 && !A->rowptr) return fail(SEGSCAN_EINVAL, "rowptr is NULL");
     if (A->nnz > 0 && !A->colind) return fail(SEGSCAN_EINVAL, "colind is NULL");
@@ -572,7 +604,7 @@ void fill_info(const SegPlan& pl, segscan_launch_info_t* info) {
 
 extern "C" {
 
-int32_t segscan_abi_version(void) { return 1; }
+int32_t segscan_abi_version(void) { return 2; }
 
 void segscan_set_timing_events(void* start, void* stop) {
   g_ev_start = static_cast<hipEvent_t>(start);
@@ -600,7 +632,7 @@ segscan_status_t segscan_spmm_describe(const segscan_csr_t* A, int64_t N, segsca
   if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N = %lld out of range", (long long)N);
   if (!dtype_valid(b_dtype)) return fail(SEGSCAN_EINVAL, "unknown dtype %d", (int)b_dtype);
   SpmmPlan pl;
-  st = plan_spmm(A->nrow, A->nnz, std::max<int64_t>(N, 1), b_dtype, variant, from_c(knobs), 8, pl);
+  st = plan_spmm(A->nrow, A->nnz, A->max_row_nnz, std::max<int64_t>(N, 1), b_dtype, variant, from_c(knobs), 8, pl);
   if (st) return st;
   fill_info(pl, info);
   return ok();
@@ -640,7 +672,7 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N, const void*
   const bool accumulate = flags & SEGSCAN_ACCUMULATE;
   const int cap = aligned_cf(8, B, std::max<int64_t>(ldb, 1), dtype_size(b_dtype), C, std::max<int64_t>(ldc, 1));
   SpmmPlan pl;
-  st = plan_spmm(M, nnz, std::max<int64_t>(N, 1), b_dtype, variant, from_c(knobs), cap, pl);
+  st = plan_spmm(M, nnz, A->max_row_nnz, std::max<int64_t>(N, 1), b_dtype, variant, from_c(knobs), cap, pl);
   if (st) return st;
   if (M == 0 || N == 0) return ok();
   if (pl.ws_bytes > 0 && nnz > 0) {

@@ -36,19 +36,21 @@ void destroy(Plan* p) {
 
 extern "C" {
 
-// b_dtype: 0 fp32, 1 bf16, 2 fp16 (C is fp32, compute fp32). alg: 0 default,
-// 1/2/3 CSR_ALG1/2/3. Returns 0 on success, else a hipSPARSE / HIP error code
-// (negative for HIP errors). Descriptors, buffer size query, buffer allocation
+// b_dtype: 0 fp32 only (the vendor SpMM has no fp32-A / 16-bit-B / fp32-C form; it
+// throws for that combination instead of returning a status, so it is refused here,
+// returning -1000). alg: 0 default, 1/2/3 CSR_ALG1/2/3. Returns 0 on success, else a
+// hipSPARSE / HIP error code (negative for HIP errors). Descriptors, buffer size query, buffer allocation
 // and preprocessing happen here, outside any timed region.
 int vendor_spmm_create(int64_t M, int64_t K, int64_t nnz, const int32_t* rowptr, const int32_t* colind,
                        const float* vals, int64_t N, const void* B, int b_dtype, float* C, int alg, void* stream,
                        void** plan_out) {
   *plan_out = nullptr;
+  if (b_dtype != 0) return -1000;
   Plan* p = new Plan();
   const hipsparseSpMMAlg_t algs[4] = {HIPSPARSE_SPMM_ALG_DEFAULT, HIPSPARSE_SPMM_CSR_ALG1, HIPSPARSE_SPMM_CSR_ALG2,
                                       HIPSPARSE_SPMM_CSR_ALG3};
   p->alg = algs[(alg >= 0 && alg < 4) ? alg : 0];
-  const hipDataType bt = b_dtype == 1 ? HIP_R_16BF : b_dtype == 2 ? HIP_R_16F : HIP_R_32F;
+  const hipDataType bt = HIP_R_32F;
   hipsparseStatus_t st;
 #define VCHECK(x)                  \
   do {                             \

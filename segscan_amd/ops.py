@@ -9,6 +9,7 @@ from ctypes import byref, c_size_t
 from dataclasses import dataclass
 from typing import Optional
 
+import numpy as np
 import torch
 
 from . import _abi
@@ -61,6 +62,7 @@ class CsrMatrix:
     rowptr: torch.Tensor
     colind: torch.Tensor
     vals: Optional[torch.Tensor] = None
+    max_row_nnz: int = 0  # longest row, 0 = unknown; only guides variant="auto"
 
     @property
     def nnz(self) -> int:
@@ -68,10 +70,21 @@ class CsrMatrix:
 
     @classmethod
     def from_numpy(cls, nrow, ncol, rowptr, colind, vals=None, device="cuda") -> "CsrMatrix":
-        rp = torch.as_tensor(rowptr, dtype=torch.int32).to(device)
+        rp_h = np.asarray(rowptr)
+        max_row = int(np.diff(rp_h.astype(np.int64)).max()) if rp_h.size > 1 else 0
+        rp = torch.as_tensor(rp_h, dtype=torch.int32).to(device)
         ci = torch.as_tensor(colind, dtype=torch.int32).to(device)
         va = None if vals is None else torch.as_tensor(vals, dtype=torch.float32).to(device)
-        return cls(int(nrow), int(ncol), rp.contiguous(), ci.contiguous(), None if va is None else va.contiguous())
+        return cls(int(nrow), int(ncol), rp.contiguous(), ci.contiguous(), None if va is None else va.contiguous(),
+                   max_row)
+
+    def with_vals(self, vals: Optional[torch.Tensor]) -> "CsrMatrix":
+        return CsrMatrix(self.nrow, self.ncol, self.rowptr, self.colind, vals, self.max_row_nnz)
+
+    def measured(self) -> "CsrMatrix":
+        """The same matrix with max_row_nnz read from the device (one synchronisation)."""
+        m = int(torch.diff(self.rowptr).max()) if self.nrow > 0 else 0
+        return CsrMatrix(self.nrow, self.ncol, self.rowptr, self.colind, self.vals, m)
 
     def desc(self) -> _abi.CsrDesc:
         d = _abi.CsrDesc()
@@ -79,6 +92,7 @@ class CsrMatrix:
         d.rowptr = self.rowptr.data_ptr() if self.rowptr.numel() else None
         d.colind = self.colind.data_ptr() if self.colind.numel() else None
         d.vals = None if self.vals is None or self.vals.numel() == 0 else self.vals.data_ptr()
+        d.max_row_nnz = self.max_row_nnz
         return d
 
 

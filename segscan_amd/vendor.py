@@ -33,7 +33,8 @@ def load():
 
 
 class VendorSpmm:
-    """C = A @ B through hipsparseSpMM (row-major dense, fp32 compute)."""
+    """C = A @ B through hipsparseSpMM (row-major dense, fp32 operands and compute;
+    the library has no fp32-A / 16-bit-B form)."""
 
     ALGS = {"default": 0, "csr_alg1": 1, "csr_alg2": 2, "csr_alg3": 3}
 
@@ -41,7 +42,9 @@ class VendorSpmm:
         lib = load()
         if A.vals is None:
             raise ValueError("hipSPARSE needs explicit values")
-        dt = {torch.float32: 0, torch.bfloat16: 1, torch.float16: 2}[B.dtype]
+        if B.dtype != torch.float32 or C.dtype != torch.float32:
+            raise ValueError("the hipSPARSE comparator takes fp32 B and C only")
+        dt = 0
         self._keep = (A, B, C)
         h = c_void_p()
         st = lib.vendor_spmm_create(A.nrow, A.ncol, A.nnz, A.rowptr.data_ptr(), A.colind.data_ptr(),

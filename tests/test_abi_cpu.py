@@ -40,15 +40,16 @@ def test_library_exports_every_declared_symbol(lib):
 
 
 def test_version_and_status_strings(lib):
-    assert lib.segscan_abi_version() == 1
+    assert lib.segscan_abi_version() == 2
     for s in range(5):
         assert lib.segscan_status_string(s).decode().startswith("SEGSCAN_")
 
 
-def _desc(nrow=10, ncol=10, nnz=20, rowptr=1, colind=1, vals=None):
+def _desc(nrow=10, ncol=10, nnz=20, rowptr=1, colind=1, vals=None, max_row=0):
     d = _abi.CsrDesc()
     d.nrow, d.ncol, d.nnz = nrow, ncol, nnz
     d.rowptr, d.colind, d.vals = rowptr, colind, vals
+    d.max_row_nnz = max_row
     return d
 
 
@@ -66,6 +67,8 @@ def _spmm(lib, desc, N=8, B=0x1000, ldb=8, C=0x2000, ldc=8, variant=_abi.SPMM_EB
     (dict(desc=_desc(colind=None)), _abi.EINVAL),
     (dict(desc=_desc(nrow=0, nnz=5)), _abi.EINVAL),
     (dict(desc=_desc(nnz=2**31)), _abi.EINVAL),
+    (dict(desc=_desc(max_row=21)), _abi.EINVAL),
+    (dict(desc=_desc(max_row=-1)), _abi.EINVAL),
     (dict(desc=_desc(), N=-1), _abi.EINVAL),
     (dict(desc=_desc(), B=None), _abi.EINVAL),
     (dict(desc=_desc(), C=None), _abi.EINVAL),
@@ -120,9 +123,9 @@ def test_backward_validation(lib):
     assert lib.segscan_check_csr(byref(d), None, None) == _abi.EINVAL
 
 
-def _describe(lib, nrow, nnz, N, variant, dtype=_abi.F32, **knobs):
+def _describe(lib, nrow, nnz, N, variant, dtype=_abi.F32, max_row=0, **knobs):
     info = _abi.LaunchInfo()
-    d = _desc(nrow=nrow, ncol=nrow, nnz=nnz)
+    d = _desc(nrow=nrow, ncol=nrow, nnz=nnz, max_row=max_row)
     kp = byref(_abi.Knobs.make(**knobs)) if knobs else None
     st = lib.segscan_spmm_describe(byref(d), N, dtype, variant, kp, byref(info))
     assert st == _abi.OK, lib.segscan_last_error().decode()
@@ -164,6 +167,39 @@ def test_describe_auto_is_concrete(lib):
     info = _describe(lib, 262144, 4194304, 64, _abi.SPMM_AUTO)
     assert info.variant in range(4)
     assert info.thread_per_block % 64 == 0 and info.thread_per_block <= 1024
+
+
+@pytest.mark.parametrize("nrow,nnz,max_row,N,variant,group,tnz", [
+    # even rows (longest <= 4 x mean + 32): one group per row
+    (16384, 65536, 14, 4, _abi.SPMM_RBPR, 4, None),      # mean 4 -> groups of 4 lanes
+    (16384, 262144, 40, 4, _abi.SPMM_RBPR, 8, None),     # mean 16 -> 8
+    (16384, 1048576, 96, 8, _abi.SPMM_RBPR, 16, None),   # mean 64 -> 16
+    (16384, 524288, 60, 16, _abi.SPMM_RBPR, 16, None),   # mean 32, N 16 -> still a group per row
+    (16384, 65536, 14, 16, _abi.SPMM_RBSR, None, None),
+    (16384, 65536, 14, 64, _abi.SPMM_RBSR, None, None),
+    # skewed or unknown: element-balanced, thread_nz = nnz / 2^18 within [64 / group, 32]
+    (16384, 65536, 0, 64, _abi.SPMM_EBSR, 16, 4),
+    (65536, 588341, 65536, 64, _abi.SPMM_EBSR, 16, 4),
+    (262144, 4194304, 20000, 64, _abi.SPMM_EBSR, 16, 16),
+    (262144, 4194304, 20000, 128, _abi.SPMM_EBSR, 32, 16),
+    (65536, 1048576, 20000, 16, _abi.SPMM_EBSR, 16, 4),
+    (65536, 1048576, 20000, 4, _abi.SPMM_EBSR, 4, 16),
+    (262144, 2**28, 20000, 64, _abi.SPMM_EBSR, 16, 32),
+])
+def test_describe_auto_follows_row_hint(lib, nrow, nnz, max_row, N, variant, group, tnz):
+    info = _describe(lib, nrow, nnz, N, _abi.SPMM_AUTO, max_row=max_row)
+    assert info.variant == variant
+    if group is not None:
+        assert info.group_size == group
+    if tnz is not None:
+        assert info.thread_nz == tnz
+    if variant == _abi.SPMM_EBSR:  # about 16 waves per CU on 256 CUs, owner stores
+        assert info.grid_x <= 1024 and info.write_policy == _abi.WRITE_OWNER and info.chunk >= 64
+
+
+def test_describe_auto_bf16_keeps_four_columns_per_lane(lib):
+    info = _describe(lib, 262144, 4194304, 128, _abi.SPMM_AUTO, dtype=_abi.BF16, max_row=20000)
+    assert (info.variant, info.group_size, info.coarsen_factor) == (_abi.SPMM_EBSR, 32, 4)
 
 
 def test_workspace_size_query(lib):
