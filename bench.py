@@ -33,6 +33,23 @@ KERNEL_OF_VARIANT = {0: "pr_eb_kernel", 1: "pr_rb_kernel", 2: "sr_kernel", 3: "r
                      4: "pr_eb_kernel", 5: "pr_rb_kernel", 6: "sr_kernel", 7: "rbsr_kernel"}
 
 
+COUNTER_SUMMARY = ROOT / "profiles" / "spmm_powerlaw262k_N64_f32_hbm_counters.txt"
+
+
+def counted_traffic(variant_id: int, knobs: dict | None, N: int) -> dict | None:
+    """HBM bytes per launch of the main kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, copy-calibrated), used only when
+    that summary was taken on this workload at the same variant and knobs."""
+    if N != 64 or not COUNTER_SUMMARY.exists():
+        return None
+    for line in COUNTER_SUMMARY.read_text().splitlines():
+        if line.startswith("bench-readable:"):
+            rec = json.loads(line.split(":", 1)[1])
+            if rec.get("kernel") == KERNEL_OF_VARIANT.get(variant_id) and rec.get("knobs") == knobs:
+                return rec
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -252,6 +269,7 @@ def main():
         log(f"cpu baseline: {cpu_s * 1e3:.1f} ms ({cpu['value']} GB/s), host {os.cpu_count()} cpus")
 
     achieved = b_alg(4) / (kern_ms * 1e-3) / 1e9
+    counted = counted_traffic(info32["variant"], k32, N)
     line = {
         "metric": "SpMM GB/s (% HBM peak) + fwd+bwd ms, N=64: tuned bf16 vs plain fp32",
         "value": round(value, 1),
@@ -271,7 +289,11 @@ def main():
                                                       "thread_per_block", "grid_x", "grid_y", "write_policy")}},
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if counted is None else counted["traffic_bytes"],
+                     "traffic_over_algorithmic": None if counted is None else
+                     round(counted["traffic_bytes"] / b_alg(4), 2),
+                     "traffic_source": None if counted is None else str(COUNTER_SUMMARY.relative_to(ROOT)),
                      "kernel": KERNEL_OF_VARIANT[info32["variant"]],
                      "kernel_us": round(kern_ms * 1e3, 2), "algorithmic_bytes": b_alg(4)},
         "cpu_baseline": cpu,

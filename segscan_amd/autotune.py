@@ -286,16 +286,21 @@ def lookup(table: dict, kind: str, dtype, N: int, stats: dict, ncol: int | None 
     return None
 
 
-def fit_table(rows: list[dict], kind: str = "spmm", exclude: str | None = None) -> dict:
+def fit_table(rows: list[dict], kind: str = "spmm", exclude: str | None = None, margin: float = 1.03,
+              min_matrices: int = 2) -> dict:
     """Bucket entries from sweep rows (segscan_amd.sweep run output): per (dtype,
     N bucket, skew class, size class) the action with the lowest geometric mean of
-    time / per-matrix best among the actions timed on every matrix of the bucket.
+    time / per-matrix best among the actions timed on every matrix of the bucket,
+    kept only where it beats the library's built-in "auto" choice on those
+    matrices by more than ``margin`` and fitted on at least ``min_matrices``
+    matrices (elsewhere lookup falls back to "auto").
     ``exclude`` leaves one matrix out (for leave-one-out evaluation)."""
     families = ("ebpr", "rbpr", "ebsr", "rbsr")
     groups: dict = {}
+    auto: dict = {}
     spec: dict = {}
     for r in rows:
-        if r["dataset"] == exclude or r.get("family") not in families:
+        if r["dataset"] == exclude or (r.get("family") not in families and r.get("action") != "auto"):
             continue
         t = r.get("ms")
         if t is None or not np.isfinite(t) or t <= 0:
@@ -303,16 +308,24 @@ def fit_table(rows: list[dict], kind: str = "spmm", exclude: str | None = None) 
         stats = {"rows": r["rows"], "nnz": r["nnz"], "max": r["max_row"],
                  "mean": r["nnz"] / max(1, r["rows"])}
         key = table_key(kind, r["dtype"], r["N"], stats)
+        if r["action"] == "auto":
+            auto.setdefault(key, {})[r["dataset"]] = float(t)
+            continue
         groups.setdefault(key, {}).setdefault(r["dataset"], {})[r["action"]] = float(t)
         spec[r["action"]] = r
     table = {}
     for key, per_ds in groups.items():
         common = set.intersection(*(set(v) for v in per_ds.values()))
-        if not common:
+        if not common or len(per_ds) < min_matrices:
             continue
         best = {d: min(v.values()) for d, v in per_ds.items()}
         score = {a: float(np.exp(np.mean([np.log(per_ds[d][a] / best[d]) for d in per_ds]))) for a in common}
         a = min(sorted(score), key=score.get)
+        au = auto.get(key, {})
+        if au and all(d in au for d in per_ds):
+            auto_score = float(np.exp(np.mean([np.log(au[d] / best[d]) for d in per_ds])))
+            if score[a] * margin >= auto_score:
+                continue
         r = spec[a]
         res = json.loads(r["resolved"]) if r.get("resolved") else {}
         table[key] = {"variant": res.get("variant", r.get("variant")), "knobs": r.get("params") or res.get("knobs"),

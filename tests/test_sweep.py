@@ -164,6 +164,13 @@ def test_fit_table_and_leave_one_out():
     # leave-one-out picks: d1 -> a2 (2.0), d2 -> a2 (1.5), d3 -> a1 (2.5)
     loo = sweep.analyse(rows)["N64_f32"]["table_loo_vs_best"]
     assert loo["mean"] == pytest.approx(2.0) and loo["max"] == pytest.approx(2.5) and loo["argmax"] == "d3"
+    # an entry must beat the built-in "auto" by the margin, and rest on >= 2 matrices
+    with_auto = list(rows)
+    for d, t in (("d1", 1.0), ("d2", 1.0), ("d3", 1.1)):  # auto gmean 1.1^(1/3) = 1.032 < 1.357
+        with_auto.append({**[r for r in rows if r["dataset"] == d][0], "action": "auto", "family": "select",
+                          "kind": "auto", "ms": t})
+    assert autotune.fit_table(with_auto) == {}
+    assert autotune.fit_table(rows, exclude="d1", min_matrices=3) == {}
 
 
 def test_lookup_prefers_matrix_entry_and_resolves_grid():
