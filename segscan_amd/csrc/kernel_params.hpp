@@ -120,9 +120,49 @@ hipError_t launch_partition_rows(const int32_t* rowptr, int M, int64_t nnz, int6
                                  int32_t* chunk_row, int32_t* chunk_prev, hipStream_t s);
 hipError_t launch_carry_fixup(const int32_t* carry_row, const float* carry_val, int64_t nchunks, int N,
                               float* C, int64_t ldc, hipStream_t s);
+// CSR -> CSC transpose as a stable LSD radix sort of the column indices, carrying
+// (row << 32 | position) per entry. Tiles of kTransposeTile entries; `passes`
+// digits of `dbits` bits cover the `bits` bits of the largest column index.
+constexpr int64_t kTransposeTile = 4096;  // 256 lanes x 16 entries
+
+struct TransposePlan {
+  int bits = 1, passes = 1, dbits = 1, buckets = 2;
+  int64_t ntiles = 0, hist_n = 0, scan_blocks = 0;
+  size_t off_keys_a = 0, off_keys_b = 0, off_pay_a = 0, off_pay_b = 0, off_hist = 0, off_offs = 0,
+         off_bsums = 0, bytes = 0;
+};
+
+inline TransposePlan plan_transpose(int64_t K, int64_t nnz) {
+  auto align256 = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
+  TransposePlan t;
+  int64_t top = K > 1 ? K - 1 : 1;
+  t.bits = 0;
+  while (top > 0) {
+    ++t.bits;
+    top >>= 1;
+  }
+  t.passes = (t.bits + 8) / 9;                      // digits of at most 9 bits
+  t.dbits = (t.bits + t.passes - 1) / t.passes;
+  t.buckets = 1 << t.dbits;
+  t.ntiles = (nnz + kTransposeTile - 1) / kTransposeTile;
+  t.hist_n = static_cast<int64_t>(t.buckets) * t.ntiles;
+  t.scan_blocks = (t.hist_n + 1023) / 1024;
+  const size_t n = static_cast<size_t>(nnz > 0 ? nnz : 1);
+  size_t off = 0;
+  t.off_keys_a = off; off += align256(n * 4);
+  t.off_keys_b = off; off += align256(n * 4);
+  t.off_pay_a = off;  off += align256(n * 8);
+  t.off_pay_b = off;  off += align256(n * 8);
+  t.off_hist = off;   off += align256(static_cast<size_t>(t.hist_n + 1) * 4);
+  t.off_offs = off;   off += align256(static_cast<size_t>(t.hist_n + 1) * 4);
+  t.off_bsums = off;  off += align256(static_cast<size_t>(t.scan_blocks + 1) * 4);
+  t.bytes = off;
+  return t;
+}
+
 hipError_t launch_transpose(const int32_t* rowptr, const int32_t* colind, const float* vals, int M, int K,
                             int64_t nnz, int32_t* colptr, int32_t* rowind, int32_t* perm, float* vals_t,
-                            int32_t* counts, int32_t* block_sums, int64_t nblocks, hipStream_t s);
+                            void* workspace, const TransposePlan& tp, hipStream_t s);
 hipError_t launch_sddmm(VType vt, const int32_t* rowptr, const int32_t* colind, int M, int64_t nnz, int N,
                         const void* X, int64_t ld_x, const void* Y, int64_t ld_y, float* out, hipStream_t s);
 hipError_t launch_segment_gather(const float* ddst, int64_t ld_ddst, const int32_t* index, int64_t E, int N,

@@ -98,21 +98,164 @@ hipError_t launch_carry_fixup(const int32_t* carry_row, const float* carry_val, 
 }
 
 // ---- CSR -> CSC transpose ---------------------------------------------------------------
-// 1. column histogram; 2. exclusive scan -> colptr; 3. scatter every nnz position
-// to its column with an atomic cursor (order within a column arbitrary);
-// 4. rank each position among its column's positions (count of smaller ones)
-// and place it there, which restores CSR order within every column — the same
-// result as a stable sort by column; 5. row index and value of every entry.
+// A stable LSD radix sort of the column indices. Every entry carries
+// (row << 32 | position); CSR order is row-major, so a stable sort by column
+// leaves the rows of each column ascending — the reference's counting-sort
+// transpose (dataloader.hpp:123-144). No global atomics: tile histograms in LDS,
+// a device-wide scan of the (digit-major) tile histograms, then a stable scatter
+// in which equal digits are ranked inside a wave by ballots and across the
+// block's waves and rounds through LDS counters. The last pass writes perm,
+// rowind and the permuted values directly; colptr is a lower bound per column
+// over the sorted keys.
 
-__global__ void col_histogram_kernel(const int32_t* __restrict__ colind, int64_t nnz, int32_t* __restrict__ counts) {
+// One 64-ary search per wave (the row of its first entry), then the wave's
+// rowptr window resolves every lane (wave_row_lookup).
+__global__ void transpose_init_kernel(const int32_t* __restrict__ rowptr, int M, int64_t nnz,
+                                      uint64_t* __restrict__ pay) {
+  const int lane = threadIdx.x & 63;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < nnz; p += stride)
-    atomicAdd(counts + colind[p], 1);
+  for (int64_t wbase = static_cast<int64_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63); wbase < nnz;
+       wbase += stride) {
+    const int64_t p = wbase + lane;
+    const int e = static_cast<int>(p < nnz ? p : nnz - 1);
+    const int r0 = static_cast<int>(wave_search_gt(rowptr, 1, M, wbase)) - 1;  // rowptr[M] = nnz > wbase
+    const uint32_t r = static_cast<uint32_t>(wave_row_lookup(rowptr, M, r0, e));
+    if (p < nnz) pay[p] = (static_cast<uint64_t>(r) << 32) | static_cast<uint32_t>(p);
+  }
+}
+
+constexpr int kRadixThreads = 256;
+constexpr int kRadixItems = static_cast<int>(kTransposeTile / kRadixThreads);  // 16
+constexpr int kRadixMaxBuckets = 512;
+
+__global__ void __launch_bounds__(kRadixThreads)
+radix_upsweep_kernel(const int32_t* __restrict__ keys, int64_t nnz, int shift, int mask, int buckets, int64_t ntiles,
+                     int32_t* __restrict__ hist) {
+  __shared__ int32_t cnt[kRadixMaxBuckets];
+  for (int b = threadIdx.x; b < buckets; b += kRadixThreads) cnt[b] = 0;
+  __syncthreads();
+  const int64_t tile = blockIdx.x;
+  const int64_t base = tile * kTransposeTile;
+  for (int j = 0; j < kRadixItems; ++j) {
+    const int64_t idx = base + static_cast<int64_t>(j) * kRadixThreads + threadIdx.x;
+    if (idx < nnz) atomicAdd(&cnt[(keys[idx] >> shift) & mask], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < buckets; b += kRadixThreads) hist[static_cast<int64_t>(b) * ntiles + tile] = cnt[b];
+}
+
+// Stable scatter of one tile. The tile's 16 entries per lane are loaded up
+// front; 16 rounds then rank each entry among equal digits (ballots inside a
+// wave, double-buffered per-wave counts across waves and rounds) and place it
+// in LDS in digit order; finally consecutive lanes copy consecutive LDS slots
+// out, so each digit's run of the tile is written as one contiguous stretch.
+// LAST: write perm, rowind and the permuted values instead of the next pass's
+// payload.
+template <bool LAST>
+__global__ void __launch_bounds__(kRadixThreads)
+radix_downsweep_kernel(const int32_t* __restrict__ keys_in, const uint64_t* __restrict__ pay_in, int64_t nnz,
+                       int shift, int mask, int dbits, int buckets, int64_t ntiles, const int32_t* __restrict__ hist,
+                       const int32_t* __restrict__ offs, int32_t* __restrict__ keys_out, uint64_t* __restrict__ pay_out,
+                       int32_t* __restrict__ perm, int32_t* __restrict__ rowind, const float* __restrict__ vals,
+                       float* __restrict__ vals_t) {
+  constexpr int kWaves = kRadixThreads / 64;
+  __shared__ int32_t s_base[kRadixMaxBuckets];    // global start of each digit's run of this tile
+  __shared__ int32_t s_lstart[kRadixMaxBuckets];  // tile-local start of each digit's run
+  __shared__ int32_t s_run[kRadixMaxBuckets];     // entries of each digit placed in earlier rounds
+  __shared__ int16_t s_wc[2][kWaves][kRadixMaxBuckets];
+  __shared__ int32_t s_key[kTransposeTile];
+  __shared__ uint64_t s_pay[kTransposeTile];
+  const int64_t tile = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  for (int b = threadIdx.x; b < buckets; b += kRadixThreads) {
+    s_base[b] = offs[static_cast<int64_t>(b) * ntiles + tile];
+    s_run[b] = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      s_wc[0][w][b] = 0;
+      s_wc[1][w][b] = 0;
+    }
+  }
+  if (wave == 0) {  // exclusive scan of this tile's digit counts
+    const int per = (buckets + 63) / 64;
+    const int b0 = lane * per;
+    int sum = 0;
+    for (int b = b0; b < b0 + per && b < buckets; ++b) sum += hist[static_cast<int64_t>(b) * ntiles + tile];
+    int incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(incl, off);
+      if (lane >= off) incl += t;
+    }
+    int run = incl - sum;
+    for (int b = b0; b < b0 + per && b < buckets; ++b) {
+      s_lstart[b] = run;
+      run += hist[static_cast<int64_t>(b) * ntiles + tile];
+    }
+  }
+  const uint64_t below = (1ull << lane) - 1ull;
+  const int64_t base = tile * kTransposeTile;
+  const int tile_n = static_cast<int>(nnz - base < kTransposeTile ? nnz - base : kTransposeTile);
+  int32_t key[kRadixItems];
+  uint64_t pay[kRadixItems];
+#pragma unroll
+  for (int j = 0; j < kRadixItems; ++j) {
+    const int i = j * kRadixThreads + threadIdx.x;
+    key[j] = i < tile_n ? keys_in[base + i] : 0;
+    pay[j] = i < tile_n ? pay_in[base + i] : 0ull;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kRadixItems; ++j) {
+    const int buf = j & 1;
+    const bool valid = j * kRadixThreads + static_cast<int>(threadIdx.x) < tile_n;
+    const int d = (key[j] >> shift) & mask;
+    // lanes of this wave holding the same digit
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < dbits; ++b) {
+      const bool bit = (d >> b) & 1;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const int rank_w = __popcll(peers & below);
+    if (valid && rank_w == 0) s_wc[buf][wave][d] = static_cast<int16_t>(__popcll(peers));
+    __syncthreads();  // this round's counts visible; the previous round's s_run update done
+    if (valid) {
+      int slot = s_lstart[d] + s_run[d] + rank_w;
+      for (int w = 0; w < wave; ++w) slot += s_wc[buf][w][d];
+      s_key[slot] = key[j];
+      s_pay[slot] = pay[j];
+    }
+    __syncthreads();  // every lane has read s_run and this round's counts
+    for (int b = threadIdx.x; b < buckets; b += kRadixThreads) {
+      int add = 0;
+      for (int w = 0; w < kWaves; ++w) {
+        add += s_wc[buf][w][b];
+        s_wc[buf][w][b] = 0;
+      }
+      s_run[b] += add;
+    }
+  }
+  __syncthreads();  // the tile sits in LDS in digit order
+  for (int slot = threadIdx.x; slot < tile_n; slot += kRadixThreads) {
+    const int32_t k = s_key[slot];
+    const uint64_t v = s_pay[slot];
+    const int d = (k >> shift) & mask;
+    const int64_t pos = s_base[d] + (slot - s_lstart[d]);
+    keys_out[pos] = k;
+    if constexpr (LAST) {
+      const int32_t pidx = static_cast<int32_t>(v & 0xffffffffull);
+      perm[pos] = pidx;
+      rowind[pos] = static_cast<int32_t>(v >> 32);
+      if (vals_t) vals_t[pos] = vals ? vals[pidx] : 1.0f;
+    } else {
+      pay_out[pos] = v;
+    }
+  }
 }
 
 // Block-local inclusive scan of 1024 entries per block (4 per thread); writes
-// exclusive prefixes to out[i+1] (out[0] written by the caller's finish pass)
-// and the block total to block_sums.
+// exclusive prefixes to out[i+1] (out[0] written by the finish pass) and the
+// block total to block_sums.
 __global__ void scan_block_kernel(const int32_t* __restrict__ in, int64_t n, int32_t* __restrict__ out,
                                   int32_t* __restrict__ block_sums) {
   __shared__ int32_t s[1024];
@@ -134,15 +277,27 @@ __global__ void scan_block_kernel(const int32_t* __restrict__ in, int64_t n, int
   if (threadIdx.x == 0) block_sums[blockIdx.x] = s[1023];
 }
 
-// Serial scan of the block totals (one thread; nblocks = ceil(K / 1024) is small).
-__global__ void scan_block_sums_kernel(int32_t* __restrict__ block_sums, int64_t nblocks) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    int32_t run = 0;
-    for (int64_t b = 0; b < nblocks; ++b) {
-      const int32_t v = block_sums[b];
-      block_sums[b] = run;
-      run += v;
+// Exclusive scan of the block totals in place, one workgroup of 1024 lanes
+// walking them 1024 at a time with a running carry.
+__global__ void __launch_bounds__(1024) scan_block_sums_kernel(int32_t* __restrict__ block_sums, int64_t nblocks) {
+  __shared__ int32_t s[1024];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  for (int64_t base = 0; base < nblocks; base += 1024) {
+    const int64_t i = base + threadIdx.x;
+    const int32_t v = i < nblocks ? block_sums[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
     }
+    if (i < nblocks) block_sums[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
   }
 }
 
@@ -153,60 +308,71 @@ __global__ void scan_finish_kernel(int32_t* __restrict__ out, int64_t n, const i
   if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
 }
 
-__global__ void col_scatter_kernel(const int32_t* __restrict__ colind, int64_t nnz, const int32_t* __restrict__ colptr,
-                                   int32_t* __restrict__ cursor, int32_t* __restrict__ tmp) {
+// colptr[c] = first sorted position whose key is >= c (c <= K; keys < K).
+__global__ void colptr_from_keys_kernel(const int32_t* __restrict__ keys, int64_t nnz, int K,
+                                        int32_t* __restrict__ colptr) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < nnz; p += stride) {
-    const int c = colind[p];
-    const int q = colptr[c] + atomicAdd(cursor + c, 1);
-    tmp[q] = static_cast<int32_t>(p);
+  for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c <= K; c += stride) {
+    int64_t lo = 0, hi = nnz;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (keys[mid] < c) lo = mid + 1;
+      else hi = mid;
+    }
+    colptr[c] = static_cast<int32_t>(lo);
   }
 }
 
-__global__ void col_rank_kernel(const int32_t* __restrict__ colind, int64_t nnz, const int32_t* __restrict__ colptr,
-                                const int32_t* __restrict__ tmp, int32_t* __restrict__ perm) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz; q += stride) {
-    const int32_t pos = tmp[q];
-    const int c = colind[pos];
-    const int lo = colptr[c], hi = colptr[c + 1];
-    int rank = 0;
-    for (int i = lo; i < hi; ++i) rank += (tmp[i] < pos) ? 1 : 0;
-    perm[lo + rank] = pos;
-  }
-}
-
-__global__ void col_finish_kernel(const int32_t* __restrict__ rowptr, int M, const float* __restrict__ vals,
-                                  int64_t nnz, const int32_t* __restrict__ perm, int32_t* __restrict__ rowind,
-                                  float* __restrict__ vals_t) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz; q += stride) {
-    const int32_t pos = perm[q];
-    rowind[q] = row_of_nnz(rowptr, M, pos);
-    if (vals_t) vals_t[q] = vals ? vals[pos] : 1.0f;
-  }
+static hipError_t launch_exclusive_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* bsums, int64_t nblocks,
+                                 hipStream_t s) {
+  scan_block_kernel<<<dim3(static_cast<unsigned>(nblocks)), 256, 0, s>>>(in, n, out, bsums);
+  scan_block_sums_kernel<<<1, 1024, 0, s>>>(bsums, nblocks);
+  scan_finish_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(out, n, bsums);
+  return hipGetLastError();
 }
 
 hipError_t launch_transpose(const int32_t* rowptr, const int32_t* colind, const float* vals, int M, int K,
                             int64_t nnz, int32_t* colptr, int32_t* rowind, int32_t* perm, float* vals_t,
-                            int32_t* counts, int32_t* block_sums, int64_t nblocks, hipStream_t s) {
-  if (K == 0) {  // no columns: colptr = {0}, and validation guarantees nnz == 0
-    fill_i32_kernel<<<1, 64, 0, s>>>(colptr, 1, 0);
+                            void* workspace, const TransposePlan& tp, hipStream_t s) {
+  if (nnz == 0 || K == 0) {  // every column empty (K == 0 implies nnz == 0 by validation)
+    fill_i32_kernel<<<grid_for(static_cast<int64_t>(K) + 1, kBlock), kBlock, 0, s>>>(colptr, K + 1, 0);
     return hipGetLastError();
   }
-  // counts doubles as the scatter cursor; tmp lives in rowind until the finish pass.
-  fill_i32_kernel<<<grid_for(K, kBlock), kBlock, 0, s>>>(counts, K, 0);
-  if (nnz > 0) col_histogram_kernel<<<grid_for(nnz, kBlock), kBlock, 0, s>>>(colind, nnz, counts);
-  scan_block_kernel<<<dim3(static_cast<unsigned>(nblocks)), 256, 0, s>>>(counts, K, colptr, block_sums);
-  scan_block_sums_kernel<<<1, 64, 0, s>>>(block_sums, nblocks);
-  scan_finish_kernel<<<grid_for(K, kBlock), kBlock, 0, s>>>(colptr, K, block_sums);
-  if (nnz > 0) {
-    fill_i32_kernel<<<grid_for(K, kBlock), kBlock, 0, s>>>(counts, K, 0);
-    int32_t* tmp = rowind;
-    col_scatter_kernel<<<grid_for(nnz, kBlock), kBlock, 0, s>>>(colind, nnz, colptr, counts, tmp);
-    col_rank_kernel<<<grid_for(nnz, kBlock), kBlock, 0, s>>>(colind, nnz, colptr, tmp, perm);
-    col_finish_kernel<<<grid_for(nnz, kBlock), kBlock, 0, s>>>(rowptr, M, vals, nnz, perm, rowind, vals_t);
+  char* ws = static_cast<char*>(workspace);
+  int32_t* keys_a = reinterpret_cast<int32_t*>(ws + tp.off_keys_a);
+  int32_t* keys_b = reinterpret_cast<int32_t*>(ws + tp.off_keys_b);
+  uint64_t* pay_a = reinterpret_cast<uint64_t*>(ws + tp.off_pay_a);
+  uint64_t* pay_b = reinterpret_cast<uint64_t*>(ws + tp.off_pay_b);
+  int32_t* hist = reinterpret_cast<int32_t*>(ws + tp.off_hist);
+  int32_t* offs = reinterpret_cast<int32_t*>(ws + tp.off_offs);
+  int32_t* bsums = reinterpret_cast<int32_t*>(ws + tp.off_bsums);
+  transpose_init_kernel<<<grid_for(nnz, kBlock), kBlock, 0, s>>>(rowptr, M, nnz, pay_a);
+  const int mask = tp.buckets - 1;
+  const dim3 tiles(static_cast<unsigned>(tp.ntiles));
+  const int32_t* kin = colind;
+  const uint64_t* pin = pay_a;
+  int32_t* kout = keys_b;
+  uint64_t* pout = pay_b;
+  for (int pass = 0; pass < tp.passes; ++pass) {
+    const int shift = pass * tp.dbits;
+    radix_upsweep_kernel<<<tiles, kRadixThreads, 0, s>>>(kin, nnz, shift, mask, tp.buckets, tp.ntiles, hist);
+    hipError_t e = launch_exclusive_scan(hist, tp.hist_n, offs, bsums, tp.scan_blocks, s);
+    if (e != hipSuccess) return e;
+    if (pass + 1 < tp.passes) {
+      radix_downsweep_kernel<false><<<tiles, kRadixThreads, 0, s>>>(kin, pin, nnz, shift, mask, tp.dbits, tp.buckets,
+                                                                    tp.ntiles, hist, offs, kout, pout, nullptr,
+                                                                    nullptr, nullptr, nullptr);
+      kin = kout;
+      pin = pout;
+      kout = (kout == keys_b) ? keys_a : keys_b;
+      pout = (pout == pay_b) ? pay_a : pay_b;
+    } else {
+      radix_downsweep_kernel<true><<<tiles, kRadixThreads, 0, s>>>(kin, pin, nnz, shift, mask, tp.dbits, tp.buckets,
+                                                                   tp.ntiles, hist, offs, kout, nullptr, perm, rowind,
+                                                                   vals, vals_t);
+    }
   }
+  colptr_from_keys_kernel<<<grid_for(static_cast<int64_t>(K) + 1, kBlock), kBlock, 0, s>>>(kout, nnz, K, colptr);
   return hipGetLastError();
 }
 

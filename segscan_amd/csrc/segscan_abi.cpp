@@ -925,9 +925,7 @@ segscan_status_t segscan_csr_transpose_workspace_size(const segscan_csr_t* A, si
   if (!bytes) return fail(SEGSCAN_EINVAL, "bytes is NULL");
   segscan_status_t st = validate_csr(A, false);
   if (st) return st;
-  const int64_t K = A->ncol;
-  const int64_t nblocks = ceil_div(std::max<int64_t>(K, 1), 1024);
-  *bytes = align_up(static_cast<size_t>(std::max<int64_t>(K, 1)) * 4) + align_up(static_cast<size_t>(nblocks) * 4);
+  *bytes = plan_transpose(A->ncol, A->nnz).bytes;
   return ok();
 }
 
@@ -943,12 +941,8 @@ segscan_status_t segscan_csr_transpose(const segscan_csr_t* A, int32_t* colptr, 
   if (workspace_bytes < need) return fail(SEGSCAN_EWORKSPACE, "workspace %zu bytes < %zu required", workspace_bytes, need);
   const int64_t K = A->ncol;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int64_t nblocks = ceil_div(K, 1024);
-  char* ws = static_cast<char*>(workspace);
-  int32_t* counts = reinterpret_cast<int32_t*>(ws);
-  int32_t* bsums = reinterpret_cast<int32_t*>(ws + align_up(static_cast<size_t>(K) * 4));
   hipError_t e = launch_transpose(A->rowptr, A->colind, A->vals, static_cast<int>(A->nrow), static_cast<int>(K), A->nnz,
-                                  colptr, rowind, perm, vals_t, counts, bsums, nblocks, s);
+                                  colptr, rowind, perm, vals_t, workspace, plan_transpose(K, A->nnz), s);
   return hip_status(e, "transpose");
 }
 
