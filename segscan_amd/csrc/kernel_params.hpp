@@ -128,8 +128,8 @@ constexpr int64_t kTransposeTile = 4096;  // 256 lanes x 16 entries
 struct TransposePlan {
   int bits = 1, passes = 1, dbits = 1, buckets = 2;
   int64_t ntiles = 0, hist_n = 0, scan_blocks = 0;
-  size_t off_keys_a = 0, off_keys_b = 0, off_pay_a = 0, off_pay_b = 0, off_hist = 0, off_offs = 0,
-         off_bsums = 0, bytes = 0;
+  size_t off_keys_a = 0, off_keys_b = 0, off_pay_a = 0, off_pay_b = 0, off_val_a = 0, off_val_b = 0, off_hist = 0,
+         off_offs = 0, off_bsums = 0, bytes = 0;
 };
 
 inline TransposePlan plan_transpose(int64_t K, int64_t nnz) {
@@ -153,6 +153,8 @@ inline TransposePlan plan_transpose(int64_t K, int64_t nnz) {
   t.off_keys_b = off; off += align256(n * 4);
   t.off_pay_a = off;  off += align256(n * 8);
   t.off_pay_b = off;  off += align256(n * 8);
+  t.off_val_a = off;  off += align256(n * 4);
+  t.off_val_b = off;  off += align256(n * 4);
   t.off_hist = off;   off += align256(static_cast<size_t>(t.hist_n + 1) * 4);
   t.off_offs = off;   off += align256(static_cast<size_t>(t.hist_n + 1) * 4);
   t.off_bsums = off;  off += align256(static_cast<size_t>(t.scan_blocks + 1) * 4);

@@ -149,15 +149,16 @@ radix_upsweep_kernel(const int32_t* __restrict__ keys, int64_t nnz, int shift, i
 // wave, double-buffered per-wave counts across waves and rounds) and place it
 // in LDS in digit order; finally consecutive lanes copy consecutive LDS slots
 // out, so each digit's run of the tile is written as one contiguous stretch.
-// LAST: write perm, rowind and the permuted values instead of the next pass's
-// payload.
-template <bool LAST>
+// LAST: write perm and rowind instead of the next pass's payload. VALS: the
+// entries' values travel with them (val_in: the CSR values in the first pass,
+// or NULL for all ones), ending in val_out (vals_t in the last pass).
+template <bool LAST, bool VALS>
 __global__ void __launch_bounds__(kRadixThreads)
 radix_downsweep_kernel(const int32_t* __restrict__ keys_in, const uint64_t* __restrict__ pay_in, int64_t nnz,
                        int shift, int mask, int dbits, int buckets, int64_t ntiles, const int32_t* __restrict__ hist,
                        const int32_t* __restrict__ offs, int32_t* __restrict__ keys_out, uint64_t* __restrict__ pay_out,
-                       int32_t* __restrict__ perm, int32_t* __restrict__ rowind, const float* __restrict__ vals,
-                       float* __restrict__ vals_t) {
+                       int32_t* __restrict__ perm, int32_t* __restrict__ rowind, const float* __restrict__ val_in,
+                       float* __restrict__ val_out) {
   constexpr int kWaves = kRadixThreads / 64;
   __shared__ int32_t s_base[kRadixMaxBuckets];    // global start of each digit's run of this tile
   __shared__ int32_t s_lstart[kRadixMaxBuckets];  // tile-local start of each digit's run
@@ -165,6 +166,7 @@ radix_downsweep_kernel(const int32_t* __restrict__ keys_in, const uint64_t* __re
   __shared__ int16_t s_wc[2][kWaves][kRadixMaxBuckets];
   __shared__ int32_t s_key[kTransposeTile];
   __shared__ uint64_t s_pay[kTransposeTile];
+  __shared__ float s_val[VALS ? kTransposeTile : 1];
   const int64_t tile = blockIdx.x;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -197,11 +199,13 @@ radix_downsweep_kernel(const int32_t* __restrict__ keys_in, const uint64_t* __re
   const int tile_n = static_cast<int>(nnz - base < kTransposeTile ? nnz - base : kTransposeTile);
   int32_t key[kRadixItems];
   uint64_t pay[kRadixItems];
+  float val[kRadixItems];
 #pragma unroll
   for (int j = 0; j < kRadixItems; ++j) {
     const int i = j * kRadixThreads + threadIdx.x;
     key[j] = i < tile_n ? keys_in[base + i] : 0;
     pay[j] = i < tile_n ? pay_in[base + i] : 0ull;
+    if constexpr (VALS) val[j] = (i < tile_n && val_in) ? val_in[base + i] : 1.0f;
   }
   __syncthreads();
 #pragma unroll
@@ -224,6 +228,7 @@ radix_downsweep_kernel(const int32_t* __restrict__ keys_in, const uint64_t* __re
       for (int w = 0; w < wave; ++w) slot += s_wc[buf][w][d];
       s_key[slot] = key[j];
       s_pay[slot] = pay[j];
+      if constexpr (VALS) s_val[slot] = val[j];
     }
     __syncthreads();  // every lane has read s_run and this round's counts
     for (int b = threadIdx.x; b < buckets; b += kRadixThreads) {
@@ -243,13 +248,12 @@ radix_downsweep_kernel(const int32_t* __restrict__ keys_in, const uint64_t* __re
     const int64_t pos = s_base[d] + (slot - s_lstart[d]);
     keys_out[pos] = k;
     if constexpr (LAST) {
-      const int32_t pidx = static_cast<int32_t>(v & 0xffffffffull);
-      perm[pos] = pidx;
+      perm[pos] = static_cast<int32_t>(v & 0xffffffffull);
       rowind[pos] = static_cast<int32_t>(v >> 32);
-      if (vals_t) vals_t[pos] = vals ? vals[pidx] : 1.0f;
     } else {
       pay_out[pos] = v;
     }
+    if constexpr (VALS) val_out[pos] = s_val[slot];
   }
 }
 
@@ -349,27 +353,38 @@ hipError_t launch_transpose(const int32_t* rowptr, const int32_t* colind, const 
   transpose_init_kernel<<<grid_for(nnz, kBlock), kBlock, 0, s>>>(rowptr, M, nnz, pay_a);
   const int mask = tp.buckets - 1;
   const dim3 tiles(static_cast<unsigned>(tp.ntiles));
+  const bool carry_vals = vals_t != nullptr;
+  float* val_a = reinterpret_cast<float*>(ws + tp.off_val_a);
+  float* val_b = reinterpret_cast<float*>(ws + tp.off_val_b);
   const int32_t* kin = colind;
   const uint64_t* pin = pay_a;
+  const float* vin = vals;  // NULL: all ones
   int32_t* kout = keys_b;
   uint64_t* pout = pay_b;
+  float* vout = val_b;
   for (int pass = 0; pass < tp.passes; ++pass) {
     const int shift = pass * tp.dbits;
     radix_upsweep_kernel<<<tiles, kRadixThreads, 0, s>>>(kin, nnz, shift, mask, tp.buckets, tp.ntiles, hist);
     hipError_t e = launch_exclusive_scan(hist, tp.hist_n, offs, bsums, tp.scan_blocks, s);
     if (e != hipSuccess) return e;
-    if (pass + 1 < tp.passes) {
-      radix_downsweep_kernel<false><<<tiles, kRadixThreads, 0, s>>>(kin, pin, nnz, shift, mask, tp.dbits, tp.buckets,
-                                                                    tp.ntiles, hist, offs, kout, pout, nullptr,
-                                                                    nullptr, nullptr, nullptr);
+    const bool last = pass + 1 == tp.passes;
+    float* vdst = last ? vals_t : vout;
+#define SEGSCAN_DOWNSWEEP(L, V)                                                                                  \
+  radix_downsweep_kernel<L, V><<<tiles, kRadixThreads, 0, s>>>(kin, pin, nnz, shift, mask, tp.dbits, tp.buckets, \
+                                                               tp.ntiles, hist, offs, kout, L ? nullptr : pout,   \
+                                                               L ? perm : nullptr, L ? rowind : nullptr, vin, vdst)
+    if (last && carry_vals) SEGSCAN_DOWNSWEEP(true, true);
+    else if (last) SEGSCAN_DOWNSWEEP(true, false);
+    else if (carry_vals) SEGSCAN_DOWNSWEEP(false, true);
+    else SEGSCAN_DOWNSWEEP(false, false);
+#undef SEGSCAN_DOWNSWEEP
+    if (!last) {
       kin = kout;
       pin = pout;
+      if (carry_vals) vin = vout;
       kout = (kout == keys_b) ? keys_a : keys_b;
       pout = (pout == pay_b) ? pay_a : pay_b;
-    } else {
-      radix_downsweep_kernel<true><<<tiles, kRadixThreads, 0, s>>>(kin, pin, nnz, shift, mask, tp.dbits, tp.buckets,
-                                                                   tp.ntiles, hist, offs, kout, nullptr, perm, rowind,
-                                                                   vals, vals_t);
+      vout = (vout == val_b) ? val_a : val_b;
     }
   }
   colptr_from_keys_kernel<<<grid_for(static_cast<int64_t>(K) + 1, kBlock), kBlock, 0, s>>>(kout, nnz, K, colptr);

@@ -27,6 +27,23 @@ def test_transpose_matches_oracle(gpu, oracle, seed):
         assert np.array_equal(T.t.vals.cpu().numpy(), m.vals[perm])
 
 
+@pytest.mark.parametrize("K,with_vals", [(2**20 + 3, True), (2**20 + 3, False), (1, True), (130000, True)])
+def test_transpose_radix_widths(gpu, oracle, K, with_vals):
+    """Column counts needing one, two and three radix passes, tails of partial tiles,
+    and matrices without values (the sort then carries ones)."""
+    M = 5000
+    m = synth.small_random_csr(M, K, min(0.9, 11.0 / K), seed=K % 97, empty_rows=0.2)
+    A = ops.CsrMatrix.from_numpy(m.nrow, m.ncol, m.rowptr, m.colind, m.vals if with_vals else None, device=gpu)
+    T = ops.csr_transpose(A)
+    colptr, rowind, perm = oracle.transpose(m.nrow, m.ncol, m.rowptr, m.colind)
+    assert m.nnz % 4096 != 0
+    assert np.array_equal(T.t.rowptr.cpu().numpy(), colptr)
+    assert np.array_equal(T.t.colind.cpu().numpy(), rowind)
+    assert np.array_equal(T.perm.cpu().numpy(), perm)
+    if with_vals:
+        assert np.array_equal(T.t.vals.cpu().numpy(), m.vals[perm])
+
+
 def test_transpose_dense_column(gpu, oracle):
     # every row references column 5: one column with 3000 entries
     M, K = 3000, 64
