@@ -338,11 +338,7 @@ segscan_status_t plan_spmm(int64_t M, int64_t nnz, int64_t max_row, int64_t N, s
       if (policy == SEGSCAN_WRITE_DEFAULT) policy = (pl.chunk >= 128) ? SEGSCAN_WRITE_OWNER : SEGSCAN_WRITE_LESSATOM;
       pl.policy = policy;
       pl.zero_fill = policy != SEGSCAN_WRITE_OWNER;
-      size_t off = 0;
-      pl.off_rows = off;
-      off += align_up(static_cast<size_t>(std::max<int64_t>(pl.nchunks, 1)) * 4);
-      pl.off_prev = off;
-      off += align_up(static_cast<size_t>(std::max<int64_t>(pl.nchunks, 1)) * 4);
+      size_t off = 0;  // the kernel finds chunk start rows itself; only the owner carries need space
       if (policy == SEGSCAN_WRITE_OWNER) {
         pl.off_crow = off;
         off += align_up(static_cast<size_t>(std::max<int64_t>(pl.nchunks, 1)) * 4);
@@ -746,17 +742,13 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N, const void*
       p.chunk = pl.chunk;
       p.nchunks = pl.nchunks;
       p.total_groups = pl.total_groups;
-      p.chunk_row = reinterpret_cast<int32_t*>(ws + pl.off_rows);
-      p.chunk_prev = reinterpret_cast<int32_t*>(ws + pl.off_prev);
       p.policy = pl.policy;
       p.accumulate = accumulate;
       if (pl.policy == SEGSCAN_WRITE_OWNER) {
         p.carry_row = reinterpret_cast<int32_t*>(ws + pl.off_crow);
         p.carry_val = reinterpret_cast<float*>(ws + pl.off_cval);
       }
-      e = launch_partition_rows(A->rowptr, p.M, nnz, pl.chunk, pl.nchunks, const_cast<int32_t*>(p.chunk_row),
-                                const_cast<int32_t*>(p.chunk_prev), s);
-      if (e != hipSuccess) return hip_status(e, "partition");
+      // chunk start rows are found by the kernel itself (no partition pass)
       l.shmem = static_cast<size_t>(pl.tpb) * 12;
       mark_start(s);
       e = launch_spmm_sr(vt, pl.CF, pl.G, p, l, compiled);

@@ -63,8 +63,19 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
         first_row = p.index[c0];
         prev_row = (c0 > 0) ? p.index[c0 - 1] : -1;
       } else {
-        first_row = p.chunk_row[c];
-        prev_row = p.chunk_prev[c];
+        // Rows of nnz c0 and c0 - 1: a G-ary search by the group (a binary search
+        // for groups under 8 lanes); the second search only when a row starts
+        // exactly at c0.
+        if constexpr (G >= 8) first_row = static_cast<int>(group_search_gt<G>(p.rowptr, 1, p.M, c0)) - 1;
+        else first_row = row_of_nnz(p.rowptr, p.M, c0);
+        if (c0 == 0) {
+          prev_row = -1;
+        } else if (p.rowptr[first_row] < c0) {
+          prev_row = first_row;
+        } else {
+          if constexpr (G >= 8) prev_row = static_cast<int>(group_search_gt<G>(p.rowptr, 1, p.M, c0 - 1)) - 1;
+          else prev_row = row_of_nnz(p.rowptr, p.M, c0 - 1);
+        }
       }
     }
     const bool is_carry = owner && chunk_valid && c0 > 0 && prev_row == first_row;
