@@ -60,8 +60,9 @@ typedef enum segscan_dtype {
  * PR = parallel (cross-lane) reduction, SR = sequential (per-lane) reduction.
  * The *_CG variants take every knob from segscan_knobs_t (spmm_cg.cuh); the
  * PLAIN_* variants ignore the knobs and use the fixed configuration of the
- * plain launchers in spmm.cuh. AUTO picks variant and knobs from the
- * autotuner table (or a built-in heuristic when no table entry matches). */
+ * plain launchers in spmm.cuh. AUTO picks variant and knobs with a built-in
+ * rule for gfx950 that uses segscan_csr_t.max_row_nnz (per-matrix and per-bucket
+ * tables live in the Python autotuner). */
 typedef enum segscan_spmm_variant {
   SEGSCAN_SPMM_EBPR = 0,        /* csrspmm_parreduce_nnzbalance_cg */
   SEGSCAN_SPMM_RBPR = 1,        /* csrspmm_parreduce_rowbalance_cg */
@@ -74,7 +75,8 @@ typedef enum segscan_spmm_variant {
   SEGSCAN_SPMM_AUTO = 8
 } segscan_spmm_variant_t;
 
-/* segment_coo variants (new-cg/include/segscan/segscan.cuh). */
+/* segment_coo variants (new-cg/include/segscan/segscan.cuh). AUTO resolves to
+ * the sequential kernel with owner stores and 256-row chunks. */
 typedef enum segscan_seg_variant {
   SEGSCAN_SEG_PR = 0,               /* segment_coo */
   SEGSCAN_SEG_PR_NEW = 1,           /* segment_coo_new */

@@ -429,16 +429,18 @@ segscan_status_t plan_seg(int64_t E, int64_t N, segscan_dtype_t st, int variant,
   Knobs k = user;
   int v = variant;
   if (v == SEGSCAN_SEG_AUTO) {
-    // Streaming src: a group of lanes per src row at 128-bit width, long owner chunks.
+    // Streaming src: a group of lanes per src row, at most 4 columns per lane
+    // (16-bit src: 16 lanes x 64-bit beat 8 lanes x 128-bit, 156 vs 235 us at
+    // N = 64), owner chunks of 256 rows, one chunk per group (the tuned point on
+    // the E = 4.19M, N = 64 power-law index).
     v = SEGSCAN_SEG_LESSATOM_SR;
-    int cf = std::min(vec, divisor_cf(N, vec));
+    int cf = std::min(std::min(vec, 4), divisor_cf(N, vec));
     int lanes = static_cast<int>(std::min<int64_t>(64, ceil_div(N, cf)));
     int g = 1;
     while (g < lanes) g <<= 1;
     int gf = 0;
     while ((1 << gf) < g) ++gf;
-    k = mk(gf == 0 ? -1 : gf, 5, 256, 1, 1, cf, std::max(1, 512 / g), SEGSCAN_WRITE_OWNER);
-    k.bd = std::max<int>(1, static_cast<int>(E / 2048));
+    k = mk(gf == 0 ? -1 : gf, 5, 256, 1, 1, cf, std::max(1, 256 / g), SEGSCAN_WRITE_OWNER);
   }
   pl.variant = v;
   switch (v) {

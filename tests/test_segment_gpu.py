@@ -79,6 +79,22 @@ def test_segment_float(gpu, oracle, dtype, variant):
     assert oracle.normalized_error(out, ref64, absd) <= 1e-6
 
 
+@pytest.mark.parametrize("E,N,dst_len", [(600000, 64, 40000), (70000, 300, 5000), (9000, 513, 700), (3, 7, 9)])
+def test_segment_auto_stream_shapes(gpu, oracle, E, N, dst_len):
+    """"auto" at larger shapes: many chunks with carries, column slices (N beyond 64
+    lanes x 4 columns), partial column groups, and fewer rows than one chunk;
+    bit-exact on integer values."""
+    rng = np.random.default_rng(E)
+    idx = np.sort(rng.integers(0, dst_len, size=E)).astype(np.int32)
+    src = rng.integers(-2, 3, size=(E, N)).astype(np.float32)
+    info = ops.segment_describe(E, N, dst_len, torch.float32, "auto")
+    assert info["variant"] == 4 and info["write_policy"] == 2
+    ref = oracle.segment_coo_f32(src, idx, dst_len)
+    out = ops.segment_coo(torch.from_numpy(src).to(gpu), torch.from_numpy(idx).to(gpu), dst_len,
+                          variant="auto").cpu().numpy()
+    assert np.array_equal(out, ref)
+
+
 def test_segment_accumulate(gpu, oracle):
     rng = np.random.default_rng(8)
     idx, dst_len = _indices()[1]
