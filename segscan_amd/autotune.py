@@ -158,33 +158,43 @@ def spmm_candidates(N: int, dtype: torch.dtype, stats: dict) -> list[tuple[str, 
 
 
 def segment_candidates(N: int, dtype: torch.dtype, E: int) -> list[tuple[str, dict | None]]:
+    """The six reference launchers' defaults, the built-in choice, and the
+    sequential kernel over group shapes (128-bit loads, and half-width loads over
+    twice the lanes, which 16-bit sources prefer), chunk lengths, workgroup sizes
+    and grid sizes."""
     vec = 4 if dtype == torch.float32 else 8
     cf = vec
     while cf > 1 and N % cf:
         cf //= 2
-    lanes = min(64, -(-N // cf))
-    g = 1
-    while g < lanes:
-        g *= 2
-    gf = int(math.log2(g))
     cands: list[tuple[str, dict | None]] = [
         ("segment_coo", None), ("segment_coo_new", None), ("segment_coo_sr", None),
         ("segment_coo_noshmem_sr", None), ("segment_coo_noshmem_lessatom_sr", None),
-        ("segment_coo_sorted_sr", None),
+        ("segment_coo_sorted_sr", None), ("auto", None),
     ]
+    shapes = []
+    c_ = cf
+    while c_ >= 1 and len(shapes) < 2:
+        lanes = min(64, -(-N // c_))
+        g_ = 1
+        while g_ < lanes:
+            g_ *= 2
+        shapes.append((c_, g_))
+        c_ //= 2
     # short chunks keep all groups streaming inside a narrow window of src (DRAM
     # row locality); long chunks cut the number of chunk-boundary rows
-    for tnz in (1, 2, 4, 16, 32, 64, 128):
-        chunk = tnz * g
-        nchunks = -(-E // chunk)
-        for tpb in (256, 512):
-            gpb = tpb // g
-            for waves_per_cu in (4, 8, 16):
-                blocks = max(1, min(-(-nchunks // gpb), (256 * waves_per_cu * 64) // tpb))
-                cands.append(("segment_coo_noshmem_lessatom_sr",
-                              dict(group_factor=gf if gf else -1, coarsen_factor=cf, thread_nz=tnz,
-                                   thread_per_block=tpb, block_numer=1, block_denom=max(1, E // blocks),
-                                   write_policy=2 if chunk >= 64 else 3)))
+    for c_, g in shapes:
+        gf = int(math.log2(g))
+        for tnz in (1, 2, 4, 16, 32, 64, 128):
+            chunk = tnz * g
+            nchunks = -(-E // chunk)
+            for tpb in (256, 512):
+                gpb = tpb // g
+                for waves_per_cu in (4, 8, 16):
+                    blocks = max(1, min(-(-nchunks // gpb), (256 * waves_per_cu * 64) // tpb))
+                    cands.append(("segment_coo_noshmem_lessatom_sr",
+                                  dict(group_factor=gf if gf else -1, coarsen_factor=c_, thread_nz=tnz,
+                                       thread_per_block=tpb, block_numer=1, block_denom=max(1, E // blocks),
+                                       write_policy=2 if chunk >= 64 else 3)))
     return cands
 
 
