@@ -5,8 +5,8 @@ Workload: power-law CSR A, 262144 x 262144, row degrees Zipf(2.1) clipped to
 [1, 20000] and scaled to ~4.19M nnz, columns uniform and sorted, values
 U(-1, 1), dense B 262144 x 64 U(-1, 1), seed 0 (synthetic data).
 
-One step = one autotuned fp32 SpMM C = A.B (partition, main kernel and carry
-fix-up; inputs resident in HBM). `value` is the whole step's algorithmic
+One step = one autotuned fp32 SpMM C = A.B (main kernel and carry fix-up;
+inputs resident in HBM). `value` is the whole step's algorithmic
 (compulsory-byte) bandwidth:
     B_alg = 4 (M + 1) + nnz (4 + 4) + K_used N 4 + M N 4      (bytes)
 The same line reports the plain fp32 kernels (the reference's fixed

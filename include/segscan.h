@@ -227,7 +227,7 @@ segscan_status_t segscan_check_index(const int32_t* index, int64_t E, int64_t ds
  * With non-NULL hipEvent_t handles, every later segscan_spmm_csr / segscan_segment_coo
  * call on this host thread records `start` on its stream right before its main kernel
  * and `stop` right after it, so the pair times that kernel alone (no zero fill,
- * partition or fix-up pass). NULL, NULL turns the hook off. */
+ * partition, transpose or fix-up pass). NULL, NULL turns the hook off. */
 void segscan_set_timing_events(void* start, void* stop);
 
 /* ---- Matrix Market input (host only) -------------------------------------------------------

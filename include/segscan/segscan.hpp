@@ -21,7 +21,7 @@
 // synchronise too, segscan.cuh:179). Differences: failures throw
 // std::runtime_error (the reference's SpMM launchers check nothing; its
 // segment_coo launchers exit()), values are read as float in every variant, and
-// scratch space for the partition / carry passes is cached in the descriptor.
+// scratch space (carries, ebpr tile rows) is cached in the descriptor.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -103,7 +103,7 @@ struct SpMatCsrDescr_t {
     d.max_row_nnz = max_row;
     return d;
   }
-  // Grow-only scratch for the partition / carry passes.
+  // Grow-only scratch for the carry and tile-row buffers.
   void* workspace(size_t bytes) {
     if (bytes > ws_bytes) {
       if (ws) check_hip(hipFree(ws), "hipFree");
