@@ -237,6 +237,7 @@ def main():
             "tuned_f32": {"variant": v32, "knobs": k32, "ms": round(ms_step, 5), "GBps": round(value, 1)},
             "tuned_bf16": {"variant": vbf, "knobs": kbf, "ms": round(ms_bf, 5), "kernel_ms": round(km_bf, 5),
                            "GBps_alg": round(b_alg(2) / (ms_bf * 1e-3) / 1e9, 1),
+                           "GFLOPs": round(2.0 * m.nnz * N / (ms_bf * 1e-3) / 1e9, 1),
                            "speedup_vs_plain_ebpr_f32": round(plain_ms / ms_bf, 3),
                            "speedup_vs_tuned_f32": round(ms_step / ms_bf, 3)},
             "hipsparse_f32_ms": vendor,
@@ -288,6 +289,7 @@ def main():
                    "launch": {k: info32[k] for k in ("group_size", "coarsen_factor", "thread_nz", "chunk",
                                                       "thread_per_block", "grid_x", "grid_y", "write_policy")}},
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
+        "GFLOPs": round(2.0 * m.nnz * N / (ms_step * 1e-3) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if counted is None else counted["traffic_bytes"],
