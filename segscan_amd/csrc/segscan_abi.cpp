@@ -415,6 +415,7 @@ VType vtype(segscan_dtype_t t) { return static_cast<VType>(static_cast<int>(t));
 struct SegPlan {
   int variant = 0;
   bool pr = true;
+  bool seq = false;  // the dedicated sequential segment kernel ("auto")
   int G = 1, T = 1, CF = 1, tnz = 1, policy = SEGSCAN_WRITE_ATOMIC;
   int X = 1, P = 1, Y = 1, slice = 1, block = 64, tpb = 64;
   int64_t grid_x = 1, grid_y = 1, ntiles = 0, chunk = 0, nchunks = 0, total_groups = 0;
@@ -441,6 +442,13 @@ segscan_status_t plan_seg(int64_t E, int64_t N, segscan_dtype_t st, int variant,
     int gf = 0;
     while ((1 << gf) < g) ++gf;
     k = mk(gf == 0 ? -1 : gf, 5, 256, 1, 1, cf, std::max(1, 256 / g), SEGSCAN_WRITE_OWNER);
+    // the chunk length and grid may be given explicitly (thread_nz, block_numer / block_denom)
+    if (user.thread_nz > 0) k.thread_nz = user.thread_nz;
+    if (user.bd > 0) {
+      k.bd = user.bd;
+      k.bn = user.bn > 0 ? user.bn : 1;
+    }
+    pl.seq = true;
   }
   pl.variant = v;
   switch (v) {
@@ -901,9 +909,13 @@ segscan_status_t segscan_segment_coo(const void* src, int64_t ld_src, segscan_dt
       p.carry_row = reinterpret_cast<int32_t*>(ws + pl.off_crow);
       p.carry_val = reinterpret_cast<float*>(ws + pl.off_cval);
     }
-    l.shmem = static_cast<size_t>(pl.tpb) * 12;
     mark_start(s);
-    e = launch_seg_sr(vt, pl.CF, pl.G, p, l, compiled);
+    if (pl.seq) {
+      e = launch_seg_seq(vt, pl.CF, pl.G, p, l, compiled);
+    } else {
+      l.shmem = static_cast<size_t>(pl.tpb) * 12;
+      e = launch_seg_sr(vt, pl.CF, pl.G, p, l, compiled);
+    }
     mark_stop(s);
     if (e == hipSuccess && compiled && pl.policy == SEGSCAN_WRITE_OWNER)
       e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, dst, ld_dst, s);
