@@ -194,26 +194,41 @@ def main():
         # tuned bf16-B
         vbf, kbf = tuned[torch.bfloat16]
         ms_bf, km_bf = timed(make_step(Bbf, vbf, kbf, C), args.steps, args.warmup)
-        # fwd + bwd: C = A.B; dB = A^T.dC; dvals = SDDMM(dC, B) on A's pattern
+        # fwd + bwd: C = A.B; dB = A^T.dC and dvals = SDDMM(dC, B) on A's pattern, both from
+        # one fused walk over the transpose (computed once per pattern, or per step in the
+        # "incl. transpose" leg); the separate leg runs the transposed SpMM and the SDDMM
+        # as two kernels.
         dC = torch.from_numpy(rng.standard_normal((m.nrow, N)).astype(np.float32)).to(dev)
         T = ops.csr_transpose(A)
+        Tt = T.t.measured().with_vals(T.t.vals)
         dB = torch.empty_like(B32)
         dvals = torch.empty(m.nnz, dtype=torch.float32, device=dev)
-        ws_t = ops.spmm_workspace(T.t, N, torch.float32, "auto")
+        ws_t = ops.spmm_workspace(Tt, N, torch.float32, "auto")
         ws_f = ops.spmm_workspace(A, N, torch.float32, v32, k32)
+        ws_b = ops.spmm_backward_workspace(T, N, torch.float32)
 
         def fwd_bwd():
             ops.spmm(A, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
-            ops.spmm(T.t, dC, variant="auto", out=dB, workspace=ws_t)
+            ops.spmm_backward(T, A.vals, dC, B32, dB=dB, dvals=dvals, workspace=ws_b)
+
+        def fwd_bwd_fixed_vals():
+            # values without gradient (a fixed normalized adjacency): dB only, values in At order
+            ops.spmm(A, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
+            ops.spmm_backward(T, None, dC, B32, dB=dB, want_dvals=False, workspace=ws_b, vals_t=Tt.vals)
+
+        def fwd_bwd_separate():
+            ops.spmm(A, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
+            ops.spmm(Tt, dC, variant="auto", out=dB, workspace=ws_t)
             ops.sddmm(A, dC, B32, out=dvals)
 
         def fwd_bwd_with_transpose():
             ops.spmm(A, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
             Tn = ops.csr_transpose(A)
-            ops.spmm(Tn.t, dC, variant="auto", out=dB, workspace=ws_t)
-            ops.sddmm(A, dC, B32, out=dvals)
+            ops.spmm_backward(Tn, A.vals, dC, B32, dB=dB, dvals=dvals, workspace=ws_b)
 
         fb_ms, _ = timed(fwd_bwd, 10, 3, kernel_events=False)
+        fbs_ms, _ = timed(fwd_bwd_separate, 10, 3, kernel_events=False)
+        fbf_ms, _ = timed(fwd_bwd_fixed_vals, 10, 3, kernel_events=False)
         fbt_ms, _ = timed(fwd_bwd_with_transpose, 5, 2, kernel_events=False)
         plain_ms = plain["plain_ebpr"]["ms"]
         # hipSPARSE on the same inputs (the reference's cuSPARSE comparison, README §5)
@@ -243,11 +258,14 @@ def main():
             "hipsparse_f32_ms": vendor,
             "hipsparse_best_over_tuned_f32": None if best_vendor is None else round(best_vendor / ms_step, 3),
             "fwd_bwd_ms": round(fb_ms, 4),
+            "fwd_bwd_ms_separate_kernels": round(fbs_ms, 4),
+            "fwd_bwd_ms_fixed_vals": round(fbf_ms, 4),
             "fwd_bwd_ms_incl_transpose": round(fbt_ms, 4),
             "gather_model_GBps_tuned_f32": round(b_gather(4) / (ms_step * 1e-3) / 1e9, 1),
         }
         log(f"plain: {plain}")
-        log(f"bf16 tuned {ms_bf * 1e3:.1f} us; fwd+bwd {fb_ms:.3f} ms ({fbt_ms:.3f} ms incl. transpose)")
+        log(f"bf16 tuned {ms_bf * 1e3:.1f} us; fwd+bwd {fb_ms:.3f} ms (separate kernels {fbs_ms:.3f} ms, "
+            f"{fbt_ms:.3f} ms incl. transpose)")
 
     # ---- CPU baseline: the reference's host check path (oracle restatement) ----
     cpu = None

@@ -26,8 +26,8 @@
  *                           :974-1008 (segment_coo_sorted_sr)
  *   segscan_csr_transpose <- the host transpose test-cg/include/dataloader/dataloader.hpp:123-144
  *                           (+ compressedRow :109-121), done on the device
- *   segscan_sddmm_csr, segscan_segment_gather, segscan_segment_counts: backward
- *                           pieces with no reference counterpart (the reference has
+ *   segscan_sddmm_csr, segscan_spmm_backward_csr, segscan_segment_gather,
+ *   segscan_segment_counts: backward pieces with no reference counterpart (the reference has
  *                           no backward); they serve the gradient of the two ops above.
  */
 #ifndef SEGSCAN_H_
@@ -209,6 +209,31 @@ segscan_status_t segscan_csr_transpose(const segscan_csr_t* A, int32_t* colptr, 
 segscan_status_t segscan_sddmm_csr(const segscan_csr_t* A, int64_t N,
                                    const void* X, int64_t ld_x, const void* Y, int64_t ld_y,
                                    segscan_dtype_t xy_dtype, float* out, void* stream);
+/* Fused backward of C = A.B (pattern fixed; values and B learnable) in one walk over
+ * At = A^T, which gathers each dC row once for both products:
+ *   dB[j, :] (=|+=) sum over p with colind[p] == j of vals[p] * dC[row(p), :]   (dB = A^T.dC)
+ *   dvals[p]  = sum_n dC[row(p), n] * B[colind[p], n]                          (SDDMM on A's pattern)
+ * At and perm are the outputs of segscan_csr_transpose(A): At.rowptr = colptr,
+ * At.colind = rowind, At.nrow = K, At.ncol = M. Values: At.vals when non-NULL (At
+ * order, e.g. the transpose's vals_t: read coalesced; `vals` is then ignored), else
+ * vals in A's (CSR) order read through perm (a random 4-byte read per entry), else
+ * all ones. dC: M x N (ld_dc), B: K x N (ld_b),
+ * both of `dtype`; dB (K x N, ld_db) and dvals (nnz) are fp32, and either may be
+ * NULL (not computed). One group of lanes covers all N columns, so N <= 512 (fp32:
+ * N <= 256 unless N is a multiple of 8); wider N returns SEGSCAN_EUNSUPPORTED (use
+ * segscan_spmm_csr on At plus segscan_sddmm_csr). Knobs: group_factor,
+ * coarsen_factor, thread_nz, thread_per_block (<= 256), block_numer / block_denom
+ * (workgroups = K * numer / denom), as for SEGSCAN_SPMM_EBSR; NULL = built-in shape.
+ * flags: SEGSCAN_ACCUMULATE adds into dB; dvals is always overwritten. Workspace
+ * (only when dB != NULL): segscan_spmm_backward_workspace_size. No counterpart in
+ * the reference, whose kernels are forward only. */
+segscan_status_t segscan_spmm_backward_workspace_size(const segscan_csr_t* At, int64_t N, segscan_dtype_t dtype,
+                                                      const segscan_knobs_t* knobs, size_t* bytes);
+segscan_status_t segscan_spmm_backward_csr(const segscan_csr_t* At, const int32_t* perm, const float* vals,
+                                           int64_t N, const void* dC, int64_t ld_dc, const void* B, int64_t ld_b,
+                                           segscan_dtype_t dtype, float* dB, int64_t ld_db, float* dvals,
+                                           const segscan_knobs_t* knobs, uint32_t flags,
+                                           void* workspace, size_t workspace_bytes, void* stream);
 /* Gather (backward of segment_coo): dsrc[e, :] = ddst[index[e], :] * scale(index[e]),
  * scale = 1 / max(counts[s], 1) when counts != NULL (mean pooling), else 1. */
 segscan_status_t segscan_segment_gather(const float* ddst, int64_t ld_ddst, const int32_t* index, int64_t E,
@@ -227,8 +252,8 @@ segscan_status_t segscan_check_index(const int32_t* index, int64_t E, int64_t ds
                                      void* stream);
 
 /* ---- measurement hook ---------------------------------------------------------------------
- * With non-NULL hipEvent_t handles, every later segscan_spmm_csr / segscan_segment_coo
- * call on this host thread records `start` on its stream right before its main kernel
+ * With non-NULL hipEvent_t handles, every later segscan_spmm_csr / segscan_segment_coo /
+ * segscan_spmm_backward_csr call on this host thread records `start` on its stream right before its main kernel
  * and `stop` right after it, so the pair times that kernel alone (no zero fill,
  * partition, transpose or fix-up pass). NULL, NULL turns the hook off. */
 void segscan_set_timing_events(void* start, void* stop);

@@ -3,9 +3,13 @@ over the same C ABI as the forward (libsegscan_hip.so); nothing here runs on the
 CPU.
 
 SpMM  C = A(vals) . B, A fixed pattern (rowptr, colind), vals and B learnable:
-    dB    = A^T . dC     CSR -> CSC transpose on the device (cached per pattern),
-                         then the same SpMM kernels on the transposed matrix
+    dB    = A^T . dC     CSR -> CSC transpose on the device (cached per pattern)
     dvals = SDDMM(dC, B) on A's pattern: dvals[p] = <dC[row(p)], B[colind[p]]>
+    When dvals is wanted, both come from one fused walk over the transpose (one
+    gather of each dC row per entry) if N fits one group of lanes (N <= 512);
+    wider N runs the SpMM kernels on the transposed matrix and the SDDMM kernel
+    separately. dB alone (values without gradient) is the SpMM on the transpose
+    with the values permuted once and cached.
 segment_coo sum / mean pooling  dst[s] = sum (or mean) of src rows with index s:
     dsrc  = gather(ddst)[index]  (divided by the segment count for mean)
 """
@@ -13,24 +17,51 @@ from __future__ import annotations
 
 import torch
 
-from . import ops
+from . import _abi, ops
 
 
 class SpmmPattern:
     """A CSR pattern with its device transpose computed once."""
 
-    def __init__(self, A: ops.CsrMatrix, variant="auto", knobs=None, variant_t="auto", knobs_t=None):
+    def __init__(self, A: ops.CsrMatrix, variant="auto", knobs=None, variant_t="auto", knobs_t=None,
+                 fused: bool = True):
         self.A = A
         T = ops.csr_transpose(A)
         self.T = ops.CscMatrix(T.t.measured(), T.perm)  # longest column guides the "auto" plan of A^T
         self.variant, self.knobs = variant, knobs
         self.variant_t, self.knobs_t = variant_t, knobs_t
+        self.fused = fused  # False: always the separate transposed SpMM + SDDMM kernels
+        self._fused = {}
+        self._vals_t, self._vals_t_key = None, None
+
+    def fused_ok(self, N: int) -> bool:
+        """Whether the fused backward covers N columns (one group of lanes per dense row)."""
+        if not self.fused:
+            return False
+        if N not in self._fused:
+            try:
+                ops.spmm_backward_workspace(self.T, N)
+                self._fused[N] = True
+            except _abi.SegscanError as exc:
+                if exc.status != _abi.EUNSUPPORTED:
+                    raise
+                self._fused[N] = False
+        return self._fused[N]
 
     def with_vals(self, vals: torch.Tensor) -> ops.CsrMatrix:
         return self.A.with_vals(vals)
 
+    def vals_in_t_order(self, vals: torch.Tensor) -> torch.Tensor:
+        """vals permuted into the transpose's order, cached while vals is unchanged
+        (same storage and version), as for a fixed normalized adjacency."""
+        key = (vals.data_ptr(), vals._version, vals.numel())
+        if self._vals_t_key != key:
+            self._vals_t = vals.index_select(0, self.T.perm.long())
+            self._vals_t_key = key
+        return self._vals_t
+
     def transposed_with_vals(self, vals: torch.Tensor) -> ops.CsrMatrix:
-        return self.T.t.with_vals(vals.index_select(0, self.T.perm.long()))
+        return self.T.t.with_vals(self.vals_in_t_order(vals))
 
 
 class SpmmFunction(torch.autograd.Function):
@@ -48,6 +79,14 @@ class SpmmFunction(torch.autograd.Function):
         pattern = ctx.pattern
         dC = dC.contiguous().float()
         dvals = dB = None
+        want_b, want_v = ctx.needs_input_grad[1], ctx.needs_input_grad[0]
+        if want_v and pattern.fused_ok(dC.size(1)):
+            # learnable values: both gradients from one walk over the transpose, the values
+            # read through the permutation inside the kernel
+            v32 = vals.detach().float().contiguous()
+            dB, dvals = ops.spmm_backward(pattern.T, v32, dC, B.detach().float().contiguous(), want_dB=want_b)
+            return dvals.to(vals.dtype), (None if dB is None else dB.to(B.dtype)), None
+        # dB alone (fixed values): the SpMM kernels on the transpose, values in its order (cached)
         if ctx.needs_input_grad[1]:
             At = pattern.transposed_with_vals(vals.detach().float())
             dB = ops.spmm(At, dC, variant=pattern.variant_t, knobs=pattern.knobs_t).to(B.dtype)

@@ -4,8 +4,9 @@ The graphs' CSR matrices are concatenated block-diagonally; one element-balanced
 SpMM launch (grid-strided, so its workgroups stay resident and walk the whole
 batch) covers every graph, whatever its size or skew. Per-graph sum pooling is
 one segment_coo over the node -> graph index (sorted by construction). The
-forward, pooling and the full backward (A^T . dH through the cached device
-transpose, SDDMM for the values, gather for the pooling) are recorded once into
+forward, pooling and the full backward (A^T . dH and the SDDMM for the values in
+one fused walk over the cached device transpose, gather for the pooling) are
+recorded once into
 a hipGraph (torch.cuda.CUDAGraph) and replayed; every buffer, workspace
 included, is allocated before capture, and no step issues hipMemset (zero
 fills are kernels).
@@ -20,7 +21,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from . import ops
+from . import _abi, ops
 from .synth import HostCsr
 
 
@@ -44,7 +45,7 @@ class BatchedGcnStep:
     dvals = SDDMM(dH_total, X) on A's pattern."""
 
     def __init__(self, batch: GraphBatch, N: int, dtype=torch.float32, variant="auto", knobs=None,
-                 variant_t="auto", knobs_t=None, seg_variant="auto", seg_knobs=None):
+                 variant_t="auto", knobs_t=None, seg_variant="auto", seg_knobs=None, fused: bool = True):
         self.b = batch
         A = batch.A
         dev = A.rowptr.device
@@ -54,6 +55,7 @@ class BatchedGcnStep:
         self.variant_t, self.knobs_t = variant_t, knobs_t
         self.seg_variant, self.seg_knobs = seg_variant, seg_knobs
         T = ops.csr_transpose(A)
+        self.T = T
         self.At = T.t.measured().with_vals(None if A.vals is None else A.vals.index_select(0, T.perm.long()))
         self.H = torch.empty((A.nrow, N), dtype=torch.float32, device=dev)
         self.P = torch.empty((batch.n_graphs, N), dtype=torch.float32, device=dev)
@@ -62,6 +64,15 @@ class BatchedGcnStep:
         self.dvals = torch.empty(max(A.nnz, 1), dtype=torch.float32, device=dev)[: A.nnz]
         self.ws_f = ops.spmm_workspace(A, N, dtype, variant, knobs)
         self.ws_t = ops.spmm_workspace(self.At, N, torch.float32, variant_t, knobs_t)
+        # fused backward (dX and dvals from one walk over the transpose) where N fits one group of lanes
+        self.fused, self.ws_b = False, None
+        if fused:
+            try:
+                self.ws_b = ops.spmm_backward_workspace(T, N, torch.float32)
+                self.fused = True
+            except _abi.SegscanError as exc:
+                if exc.status != _abi.EUNSUPPORTED:
+                    raise
         info = ops.segment_describe(A.nrow, N, batch.n_graphs, torch.float32, seg_variant, seg_knobs)
         self.ws_s = torch.empty(max(1, info["workspace_bytes"]), dtype=torch.uint8, device=dev)
         self.graph = None
@@ -74,8 +85,13 @@ class BatchedGcnStep:
         ops.segment_gather(dP, self.b.graph_index, out=self.dHt)
         if dH is not None:
             self.dHt.add_(dH)
-        ops.spmm(self.At, self.dHt, self.variant_t, self.knobs_t, out=self.dX, workspace=self.ws_t)
         Xf = X if X.dtype == torch.float32 else X.float()
+        if self.fused:
+            # the batch's values are fixed: read them in the transpose's order (cached at set-up)
+            ops.spmm_backward(self.T, None, self.dHt, Xf, dB=self.dX, dvals=self.dvals, workspace=self.ws_b,
+                              vals_t=self.At.vals)
+            return
+        ops.spmm(self.At, self.dHt, self.variant_t, self.knobs_t, out=self.dX, workspace=self.ws_t)
         ops.sddmm(A, self.dHt, Xf, out=self.dvals)
 
     def capture(self, X: torch.Tensor, dP: torch.Tensor, dH: torch.Tensor | None = None) -> torch.cuda.CUDAGraph:

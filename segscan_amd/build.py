@@ -22,6 +22,7 @@ INCLUDE = PKG.parent / "include"
 SOURCES = [
     "pr_kernels.hip",
     "sr_kernels.hip",
+    "bwd_kernels.hip",
     "rbsr_kernels.hip",
     "misc_kernels.hip",
     "segscan_abi.cpp",

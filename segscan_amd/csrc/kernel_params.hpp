@@ -80,6 +80,34 @@ struct SddmmParams {
   int64_t total_groups;
 };
 
+// Fused SpMM backward over At = A^T (dB = At.dC and dvals = SDDMM(dC, B) on A's
+// pattern from one gather of dC per entry). Groups of G lanes cover all N
+// columns (G * CF >= N) and walk chunks of `chunk` consecutive At entries.
+struct BwdParams {
+  int K;                   // rows of At (columns of A)
+  int N;
+  int64_t nnz;
+  const int32_t* rowptr;   // At: K + 1 offsets
+  const int32_t* colind;   // At: row of A of every entry
+  const int32_t* perm;     // At position -> A position
+  const float* vals_t;     // values in At order, nullable (then vals through perm)
+  const float* vals;       // A's values in A's order, nullable (all ones when vals_t is null too)
+  const void* dC;          // gathered rows, M x N
+  int64_t ld_dc;
+  const void* B;           // row j of At reads B[j], K x N
+  int64_t ld_b;
+  float* dB;               // K x N, nullable (not computed)
+  int64_t ld_db;
+  float* dvals;            // nnz, A's order, nullable (not computed)
+  int tpb;
+  int64_t chunk;
+  int64_t nchunks;
+  int64_t total_groups;
+  int32_t* carry_row;      // [nchunks] row of the chunk's carry or -1
+  float* carry_val;        // [nchunks x N]
+  int accumulate;          // dB += instead of dB =
+};
+
 // Row-balanced sequential reduction (rbsr): lanes [rows][column vectors].
 struct RbsrParams {
   int M;
@@ -116,6 +144,8 @@ hipError_t launch_seg_sr(VType vt, int cf, int g, const SrParams& p, const Launc
 // 256-lane workgroups, p.total_groups = groups in the grid).
 hipError_t launch_seg_seq(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok);
 hipError_t launch_sddmm_sr(VType vt, int cf, int g, const SddmmParams& p, const Launch& l, bool& ok);
+// Fused backward (bwd_kernels.hip): l.shmem = 12 * p.tpb, p.tpb <= 256.
+hipError_t launch_spmm_bwd(VType vt, int cf, int g, const BwdParams& p, const Launch& l, bool& ok);
 
 // Helpers (misc_kernels.hip).
 hipError_t launch_fill_zero(float* C, int64_t rows, int64_t N, int64_t ldc, hipStream_t s);

@@ -973,6 +973,179 @@ segscan_status_t segscan_sddmm_csr(const segscan_csr_t* A, int64_t N, const void
   return hip_status(e, "sddmm");
 }
 
+// Fused backward plan: one group of G lanes x CF columns covers all N columns
+// (G * CF >= N); CF <= 4 by default (as the forward's element-balanced rule),
+// widened to 8 only when N needs more than 64 lanes. Chunks of 16 tiles (16 G
+// entries) while that leaves >= 2048 chunks, else down to 64 entries: a chunk
+// boundary inside an At row costs a carry of N floats and N fix-up atomics, and
+// on a skewed transpose (131072-node power-law graph, N = 64 / 128) chunks of
+// 16 tiles ran 11 % / 10 % faster than chunks of 8 / 4 tiles; on an even one
+// (16 entries per row) 8 and 16 tiles tie. <= 1024 workgroups of 256 lanes,
+// the rest grid-strided.
+struct BwdPlan {
+  int G = 1, CF = 1, tpb = 256;
+  int64_t chunk = 0, nchunks = 0, grid = 1, total_groups = 0;
+  size_t off_crow = 0, off_cval = 0, ws_bytes = 0;
+};
+
+segscan_status_t plan_bwd(int64_t K, int64_t nnz, int64_t N, segscan_dtype_t dt, const Knobs& k, int cf_cap,
+                          BwdPlan& pl) {
+  const int vec = vec_elems(dt);
+  int cf = k.cf;
+  if (cf) {
+    if (!(cf == 1 || cf == 2 || cf == 4 || cf == 8)) return fail(SEGSCAN_EUNSUPPORTED, "coarsen_factor %d not in {1,2,4,8}", cf);
+  } else {
+    cf = std::min(std::min(vec, 4), divisor_cf(N, vec));
+    while (cf < 8 && ceil_div(N, cf) > 64 && N % (cf * 2) == 0) cf <<= 1;
+  }
+  cf = std::min(cf, cf_cap);
+  int G;
+  if (k.group_factor) {
+    const int gf = factor_value(k.group_factor);
+    if (gf > 6) return fail(SEGSCAN_EUNSUPPORTED, "group_factor %d: groups are at most 64 lanes (wave64)", gf);
+    G = 1 << gf;
+  } else {
+    G = 1;
+    while (G < 64 && static_cast<int64_t>(G) * cf < N) G <<= 1;
+  }
+  if (static_cast<int64_t>(G) * cf < N)
+    return fail(SEGSCAN_EUNSUPPORTED,
+                "fused backward needs group_size x coarsen_factor >= N (%d x %d < %lld); use the transposed SpMM "
+                "and SDDMM separately",
+                G, cf, (long long)N);
+  int tnz = k.thread_nz;
+  if (!tnz) {
+    tnz = std::max(1, 64 / G);
+    while (tnz < 16 && ceil_div(nnz, static_cast<int64_t>(2 * tnz) * G) >= 2048) tnz <<= 1;
+  }
+  if (tnz < 1 || tnz > (1 << 20)) return fail(SEGSCAN_EUNSUPPORTED, "thread_nz %d outside [1, 2^20]", tnz);
+  int tpb = k.tpb ? k.tpb : 256;
+  tpb = round_up64(std::max(tpb, G));
+  if (tpb > 256) return fail(SEGSCAN_EUNSUPPORTED, "thread_per_block %d: the fused backward takes at most 256", tpb);
+  pl.G = G;
+  pl.CF = cf;
+  pl.tpb = tpb;
+  pl.chunk = static_cast<int64_t>(tnz) * G;
+  pl.nchunks = ceil_div(std::max<int64_t>(nnz, 1), pl.chunk);
+  const int64_t gpb = tpb / G;
+  const int64_t need = std::max<int64_t>(1, ceil_div(pl.nchunks, gpb));
+  int64_t grid = std::min<int64_t>(need, 1024);
+  if (k.bn || k.bd) {
+    if (k.bn < 0 || k.bd < 0) return fail(SEGSCAN_EINVAL, "block_factor %d/%d must be positive", k.bn, k.bd);
+    Knobs w = k;
+    if (!w.bn) w.bn = 1;
+    if (!w.bd) w.bd = 1;
+    grid = std::min(need, workers(K, w));
+  }
+  if (grid > kMaxI32) return fail(SEGSCAN_EUNSUPPORTED, "grid too large");
+  pl.grid = grid;
+  pl.total_groups = grid * gpb;
+  pl.off_crow = 0;
+  pl.off_cval = align_up(static_cast<size_t>(pl.nchunks) * 4);
+  pl.ws_bytes = pl.off_cval + align_up(static_cast<size_t>(pl.nchunks) * static_cast<size_t>(std::max<int64_t>(N, 1)) * 4);
+  return SEGSCAN_OK;
+}
+
+segscan_status_t segscan_spmm_backward_workspace_size(const segscan_csr_t* At, int64_t N, segscan_dtype_t dtype,
+                                                      const segscan_knobs_t* knobs, size_t* bytes) {
+  if (!bytes) return fail(SEGSCAN_EINVAL, "bytes is NULL");
+  segscan_status_t st = validate_csr(At, false);
+  if (st) return st;
+  if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N = %lld out of range", (long long)N);
+  if (!dtype_valid(dtype)) return fail(SEGSCAN_EINVAL, "unknown dtype %d", (int)dtype);
+  BwdPlan pl;
+  st = plan_bwd(At->nrow, At->nnz, std::max<int64_t>(N, 1), dtype, from_c(knobs), 8, pl);
+  if (st) return st;
+  *bytes = pl.ws_bytes;
+  return ok();
+}
+
+segscan_status_t segscan_spmm_backward_csr(const segscan_csr_t* At, const int32_t* perm, const float* vals, int64_t N,
+                                           const void* dC, int64_t ld_dc, const void* B, int64_t ld_b,
+                                           segscan_dtype_t dtype, float* dB, int64_t ld_db, float* dvals,
+                                           const segscan_knobs_t* knobs, uint32_t flags, void* workspace,
+                                           size_t workspace_bytes, void* stream) {
+  segscan_status_t st = validate_csr(At, true);
+  if (st) return st;
+  if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N = %lld out of range", (long long)N);
+  if (!dtype_valid(dtype)) return fail(SEGSCAN_EINVAL, "unknown dtype %d", (int)dtype);
+  if (flags & ~SEGSCAN_ACCUMULATE) return fail(SEGSCAN_EINVAL, "unknown flags 0x%x", flags);
+  const int64_t K = At->nrow, nnz = At->nnz;
+  const size_t es = dtype_size(dtype);
+  if (dB && K > 0 && N > 0) {
+    if (ld_db < N) return fail(SEGSCAN_EINVAL, "ld_db %lld < N %lld", (long long)ld_db, (long long)N);
+    if (reinterpret_cast<uintptr_t>(dB) % 4) return fail(SEGSCAN_EINVAL, "dB is not 4-byte aligned");
+  }
+  if (nnz > 0) {
+    if (!perm) return fail(SEGSCAN_EINVAL, "perm is NULL");
+    if (N > 0) {
+      if (!dC || !B) return fail(SEGSCAN_EINVAL, "dC / B is NULL");
+      if (ld_dc < N || ld_b < N) return fail(SEGSCAN_EINVAL, "leading dimension < N");
+      if (reinterpret_cast<uintptr_t>(dC) % es || reinterpret_cast<uintptr_t>(B) % es)
+        return fail(SEGSCAN_EINVAL, "dC / B is misaligned");
+    }
+  }
+  const bool accumulate = flags & SEGSCAN_ACCUMULATE;
+  const int64_t ldd = dB ? std::max<int64_t>(ld_db, 1) : std::max<int64_t>(ld_dc, 1);
+  const int cap = std::min(aligned_cf(8, dC, std::max<int64_t>(ld_dc, 1), es, dB, ldd),
+                           aligned_cf(8, B, std::max<int64_t>(ld_b, 1), es, nullptr, std::max<int64_t>(ld_b, 1)));
+  BwdPlan pl;
+  st = plan_bwd(K, nnz, std::max<int64_t>(N, 1), dtype, from_c(knobs), cap, pl);
+  if (st) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (N == 0) {
+    if (dvals && nnz > 0) return hip_status(launch_fill_zero(dvals, 1, nnz, nnz, s), "dvals zero");
+    return ok();
+  }
+  if (nnz == 0) {
+    if (dB && !accumulate && K > 0) return hip_status(launch_fill_zero(dB, K, N, ld_db, s), "dB zero fill");
+    return ok();
+  }
+  if (!dB && !dvals) return ok();
+  if (dB) {
+    if (!workspace) return fail(SEGSCAN_EWORKSPACE, "workspace of %zu bytes required", pl.ws_bytes);
+    if (workspace_bytes < pl.ws_bytes)
+      return fail(SEGSCAN_EWORKSPACE, "workspace %zu bytes < %zu required", workspace_bytes, pl.ws_bytes);
+  }
+  char* ws = static_cast<char*>(workspace);
+  BwdParams p{};
+  p.K = static_cast<int>(K);
+  p.N = static_cast<int>(N);
+  p.nnz = nnz;
+  p.rowptr = At->rowptr;
+  p.colind = At->colind;
+  p.perm = perm;
+  p.vals_t = At->vals;
+  p.vals = At->vals ? nullptr : vals;
+  p.dC = dC;
+  p.ld_dc = ld_dc;
+  p.B = B;
+  p.ld_b = ld_b;
+  p.dB = dB;
+  p.ld_db = ld_db;
+  p.dvals = dvals;
+  p.tpb = pl.tpb;
+  p.chunk = pl.chunk;
+  p.nchunks = pl.nchunks;
+  p.total_groups = pl.total_groups;
+  p.carry_row = dB ? reinterpret_cast<int32_t*>(ws + pl.off_crow) : nullptr;
+  p.carry_val = dB ? reinterpret_cast<float*>(ws + pl.off_cval) : nullptr;
+  p.accumulate = accumulate;
+  Launch l;
+  l.grid = dim3(static_cast<unsigned>(pl.grid));
+  l.block = dim3(pl.tpb);
+  l.shmem = static_cast<size_t>(pl.tpb) * 12;
+  l.stream = s;
+  bool compiled = true;
+  mark_start(s);
+  hipError_t e = launch_spmm_bwd(vtype(dtype), pl.CF, pl.G, p, l, compiled);
+  mark_stop(s);
+  if (!compiled)
+    return fail(SEGSCAN_EUNSUPPORTED, "no compiled backward kernel for coarsen %d, group %d", pl.CF, pl.G);
+  if (e == hipSuccess && dB) e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, dB, ld_db, s);
+  return hip_status(e, "fused backward");
+}
+
 segscan_status_t segscan_segment_gather(const float* ddst, int64_t ld_ddst, const int32_t* index, int64_t E, int64_t N,
                                         const int32_t* counts, float* dsrc, int64_t ld_dsrc, void* stream) {
   if (E < 0 || E > kMaxI32 || N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "size out of range");

@@ -219,6 +219,55 @@ def sddmm(A: CsrMatrix, X: torch.Tensor, Y: torch.Tensor, out: Optional[torch.Te
     return out
 
 
+def spmm_backward_workspace(T: CscMatrix, N: int, dtype=torch.float32, knobs=None, device=None) -> torch.Tensor:
+    lib = _abi.load()
+    ws_bytes = c_size_t(0)
+    _abi.check(lib.segscan_spmm_backward_workspace_size(byref(T.t.desc()), N, _DTYPES[dtype], _abi.knobs_ptr(knobs),
+                                                        byref(ws_bytes)), "spmm_backward_workspace_size")
+    return torch.empty(max(1, ws_bytes.value), dtype=torch.uint8, device=device or T.t.rowptr.device)
+
+
+def spmm_backward(T: CscMatrix, vals: Optional[torch.Tensor], dC: torch.Tensor, B: torch.Tensor, want_dB: bool = True,
+                  want_dvals: bool = True, dB: Optional[torch.Tensor] = None, dvals: Optional[torch.Tensor] = None,
+                  knobs=None, accumulate: bool = False, workspace: Optional[torch.Tensor] = None, stream=None,
+                  vals_t: Optional[torch.Tensor] = None):
+    """Fused backward of C = A @ B over T = csr_transpose(A), one gather of dC per entry:
+    dB = A^T @ dC (K x N, fp32) and dvals[p] = <dC[row(p)], B[colind[p]]> (nnz, fp32, A's order).
+    vals: A's values in A's order (None = ones), read through T.perm; vals_t: the same values
+    in the transpose's order (e.g. T.t.vals while A's values are unchanged), read coalesced
+    and used instead of vals. Returns (dB or None, dvals or None)."""
+    _require_device(dC, B)
+    lib = _abi.load()
+    if dC.dtype != B.dtype:
+        raise TypeError("dC and B must share a dtype")
+    K, M, nnz = T.t.nrow, T.t.ncol, T.t.nnz
+    N = B.size(1)
+    if tuple(B.shape) != (K, N) or tuple(dC.shape) != (M, N):
+        raise ValueError(f"B must be ({K}, N) and dC ({M}, N)")
+    if want_dB and dB is None:
+        dB = torch.empty((K, N), dtype=torch.float32, device=B.device)
+    if want_dvals and dvals is None:
+        dvals = torch.empty(max(nnz, 1), dtype=torch.float32, device=B.device)[:nnz]
+    if dB is not None and (dB.dtype != torch.float32 or tuple(dB.shape) != (K, N)):
+        raise ValueError("dB must be float32 of shape (K, N)")
+    if dvals is not None and (dvals.dtype != torch.float32 or dvals.numel() != nnz):
+        raise ValueError("dvals must be float32 with nnz elements")
+    for name, v in (("vals", vals), ("vals_t", vals_t)):
+        if v is not None and (v.dtype != torch.float32 or v.numel() != nnz or not v.is_contiguous()):
+            raise ValueError(f"{name} must be contiguous float32 with nnz elements")
+    kp = _abi.knobs_ptr(knobs)
+    if workspace is None:
+        workspace = spmm_backward_workspace(T, N, B.dtype, knobs, B.device) if dB is not None else None
+    desc = T.t.with_vals(vals_t).desc()  # the At-order values, if any, travel in At.vals
+    st = lib.segscan_spmm_backward_csr(byref(desc), _ptr(T.perm), _ptr(vals), N, _ptr(dC), _leading_dim(dC, "dC"),
+                                       _ptr(B), _leading_dim(B, "B"), _dtype_code(B), _ptr(dB),
+                                       _leading_dim(dB, "dB") if dB is not None else 0, _ptr(dvals), kp,
+                                       _abi.ACCUMULATE if accumulate else 0, _ptr(workspace),
+                                       0 if workspace is None else workspace.numel(), _stream(stream))
+    _abi.check(st, "spmm_backward")
+    return dB, dvals
+
+
 def segment_counts(index: torch.Tensor, dst_len: int, stream=None) -> torch.Tensor:
     _require_device(index)
     lib = _abi.load()

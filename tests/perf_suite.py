@@ -181,6 +181,9 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "batc
                 C = ag.spmm(vals, Xg, pattern)
                 C.backward(dC)
 
+            pattern.fused = False
+            ms_sep = _time(fwd_bwd, reps=10)  # transposed SpMM + SDDMM as two kernels
+            pattern.fused = True
             ms = _time(fwd_bwd, reps=10)
             fwd = _time(lambda: ag.spmm(vals.detach(), Xg.detach(), pattern), reps=10)
             # accuracy of the gradients against float64
@@ -204,7 +207,8 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "batc
 
             pool_ms = _time(pool_fb, reps=10)
             _emit(out, {"leg": f"gcn_131072_N128_{str(dtype).split('.')[-1]}", "nnz": mg.nnz,
-                        "fwd_ms": round(fwd, 4), "fwd_bwd_ms": round(ms, 4), "transpose_ms_once": round(t_pattern, 3),
+                        "fwd_ms": round(fwd, 4), "fwd_bwd_ms": round(ms, 4), "fwd_bwd_separate_ms": round(ms_sep, 4),
+                        "transpose_ms_once": round(t_pattern, 3),
                         "meanpool_fwd_bwd_ms": round(pool_ms, 4),
                         "err_dX": oracle.normalized_error(Xg.grad.float().cpu().numpy(), dX64, dXabs),
                         "err_dvals": oracle.normalized_error(vals.grad.cpu().numpy(), dv64, dvabs)})
@@ -221,6 +225,9 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "batc
         eager_ms = _time(lambda: step.run(X, dP), reps=10)
         g = step.capture(X, dP)
         graph_ms = _time(g.replay, reps=20)
+        sep = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, variant=v, knobs=k, fused=False)
+        g_sep = sep.capture(X, dP)
+        graph_sep_ms = _time(g_sep.replay, reps=20)  # transposed SpMM + SDDMM as two kernels
         # forward-only comparison against the per-graph plain launch loop
         fwd_ms = _time(lambda: (ops.spmm(b.A, X, v, k, out=step.H, workspace=step.ws_f),
                                 ops.segment_coo(step.H, b.graph_index, b.n_graphs, out=step.P, workspace=step.ws_s)),
@@ -234,7 +241,8 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "batc
         loop_auto_ms = _time(lambda: loop_auto.run(Xf, H, P), reps=5, warmup=1)
         _emit(out, {"leg": "batched_256_graphs_N64_bf16", "nodes": csr.nrow, "nnz": csr.nnz,
                     "tuned": {"variant": v, "knobs": k}, "fwd_bwd_eager_ms": round(eager_ms, 4),
-                    "fwd_bwd_hipgraph_ms": round(graph_ms, 4), "fwd_pool_batched_ms": round(fwd_ms, 4),
+                    "fwd_bwd_hipgraph_ms": round(graph_ms, 4), "fwd_bwd_hipgraph_separate_ms": round(graph_sep_ms, 4),
+                    "fwd_pool_batched_ms": round(fwd_ms, 4),
                     "fwd_pool_per_graph_loop_plain_ms": round(loop_ms, 4),
                     "fwd_pool_per_graph_loop_auto_ms": round(loop_auto_ms, 4),
                     "speedup_batched_vs_loop": round(loop_ms / fwd_ms, 2)})

@@ -123,6 +123,34 @@ def test_backward_validation(lib):
     assert lib.segscan_check_csr(byref(d), None, None) == _abi.EINVAL
 
 
+def test_fused_backward_validation_and_plan(lib):
+    d = _desc(nrow=10, ncol=12, nnz=20)
+    f = lib.segscan_spmm_backward_csr
+    #     At       perm  vals  N  dC    ld  B     ld dtype dB    ld  dvals knobs fl  ws    bytes stream
+    assert f(byref(d), None, None, 8, 0x100, 8, 0x200, 8, 0, 0x300, 8, 0x400, None, 0, 0x500, 1 << 20, None) == _abi.EINVAL
+    assert f(byref(d), 0x10, None, 8, None, 8, 0x200, 8, 0, 0x300, 8, 0x400, None, 0, 0x500, 1 << 20, None) == _abi.EINVAL
+    assert f(byref(d), 0x10, None, 8, 0x100, 4, 0x200, 8, 0, 0x300, 8, 0x400, None, 0, 0x500, 1 << 20, None) == _abi.EINVAL
+    assert f(byref(d), 0x10, None, 8, 0x100, 8, 0x200, 8, 0, 0x300, 4, 0x400, None, 0, 0x500, 1 << 20, None) == _abi.EINVAL
+    assert f(byref(d), 0x10, None, 8, 0x100, 8, 0x200, 8, 0, 0x300, 8, 0x400, None, 2, 0x500, 1 << 20, None) == _abi.EINVAL
+    assert f(byref(d), 0x10, None, 8, 0x100, 8, 0x200, 8, 0, 0x300, 8, 0x400, None, 0, None, 0, None) == _abi.EWORKSPACE
+    # wider than one group of 64 lanes x 8 columns
+    assert f(byref(d), 0x10, None, 600, 0x100, 600, 0x200, 600, 0, 0x300, 600, 0x400, None, 0, 0x500, 1 << 30,
+             None) == _abi.EUNSUPPORTED
+    assert "coarsen_factor" in lib.segscan_last_error().decode()
+    kk = _abi.Knobs.make(thread_per_block=512)
+    assert f(byref(d), 0x10, None, 8, 0x100, 8, 0x200, 8, 0, 0x300, 8, 0x400, byref(kk), 0, 0x500, 1 << 20,
+             None) == _abi.EUNSUPPORTED
+    # workspace: one carry row index and one N-wide carry per chunk
+    n = c_size_t(0)
+    big = _desc(nrow=262144, ncol=262144, nnz=4194304)
+    assert lib.segscan_spmm_backward_workspace_size(byref(big), 64, _abi.F32, None, byref(n)) == _abi.OK
+    chunks = 4194304 // (16 * 16)  # 16 lanes x 4 columns cover N = 64; thread_nz 16 at this nnz
+    assert n.value == chunks * 4 + chunks * 64 * 4
+    kk = _abi.Knobs.make(thread_nz=2)
+    assert lib.segscan_spmm_backward_workspace_size(byref(big), 64, _abi.F32, byref(kk), byref(n)) == _abi.OK
+    assert n.value == (4194304 // 32) * 4 + (4194304 // 32) * 64 * 4
+
+
 def _describe(lib, nrow, nnz, N, variant, dtype=_abi.F32, max_row=0, **knobs):
     info = _abi.LaunchInfo()
     d = _desc(nrow=nrow, ncol=nrow, nnz=nnz, max_row=max_row)
