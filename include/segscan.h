@@ -167,7 +167,8 @@ typedef struct segscan_launch_info {
   size_t workspace_bytes;
 } segscan_launch_info_t;
 
-/* ---- SpMM: C[M x N] (=|+=) A[M x K] . B[K x N] -------------------------------------- */
+/* ---- SpMM: C[M x N] (=|+=) A[M x K] . B[K x N] --------------------------------------
+ * B row stride ldb (elements) in [N, 2^31 - 1]; C row stride ldc >= N. */
 segscan_status_t segscan_spmm_describe(const segscan_csr_t* A, int64_t N, segscan_dtype_t b_dtype,
                                        segscan_spmm_variant_t variant, const segscan_knobs_t* knobs,
                                        segscan_launch_info_t* info);
@@ -182,7 +183,7 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N,
 
 /* ---- segment_coo: dst[s, :] (=|+=) sum_{e : index[e] == s} src[e, :] ----------------
  * index: [E] int32, sorted non-decreasing, values in [0, dst_len). src: E x N, row
- * stride ld_src. dst: dst_len x N fp32, row stride ld_dst. */
+ * stride ld_src in [N, 2^31 - 1]. dst: dst_len x N fp32, row stride ld_dst. */
 segscan_status_t segscan_segment_coo_describe(int64_t E, int64_t N, int64_t dst_len, segscan_dtype_t src_dtype,
                                               segscan_seg_variant_t variant, const segscan_knobs_t* knobs,
                                               segscan_launch_info_t* info);

@@ -214,8 +214,10 @@ segscan_status_t plan_spmm(int64_t M, int64_t nnz, int64_t max_row, int64_t N, s
       while (g < lanes) g <<= 1;
       int gf = 0;
       while ((1 << gf) < g) ++gf;
+      // the largest power of two <= 1.125 nnz / 2^18 (sweep optima: 4.19M nnz -> 16,
+      // 2.1M -> 8, 1.05M -> 4), within [64 / g, 32]
       int tnz = std::max(1, 64 / g);
-      while (tnz < 32 && (static_cast<int64_t>(tnz) << 19) <= nnz) tnz <<= 1;
+      while (tnz < 32 && (static_cast<int64_t>(tnz) << 19) <= nnz + nnz / 8) tnz <<= 1;
       const int64_t chunk = static_cast<int64_t>(tnz) * g;
       const int64_t gpb = 256 / g;
       const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(ceil_div(nnz, chunk), gpb),
@@ -673,6 +675,7 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N, const void*
   if (nnz > 0 && N > 0) {
     if (!B) return fail(SEGSCAN_EINVAL, "B is NULL");
     if (ldb < N) return fail(SEGSCAN_EINVAL, "ldb %lld < N %lld", (long long)ldb, (long long)N);
+    if (ldb > kMaxI32) return fail(SEGSCAN_EUNSUPPORTED, "ldb %lld beyond 2^31 - 1", (long long)ldb);
     if (reinterpret_cast<uintptr_t>(B) % dtype_size(b_dtype)) return fail(SEGSCAN_EINVAL, "B is misaligned");
   }
   const bool accumulate = flags & SEGSCAN_ACCUMULATE;
@@ -833,6 +836,7 @@ segscan_status_t segscan_segment_coo(const void* src, int64_t ld_src, segscan_dt
     if (!src) return fail(SEGSCAN_EINVAL, "src is NULL");
     if (!index) return fail(SEGSCAN_EINVAL, "index is NULL");
     if (ld_src < N) return fail(SEGSCAN_EINVAL, "ld_src %lld < N %lld", (long long)ld_src, (long long)N);
+    if (ld_src > kMaxI32) return fail(SEGSCAN_EUNSUPPORTED, "ld_src %lld beyond 2^31 - 1", (long long)ld_src);
     if (reinterpret_cast<uintptr_t>(src) % dtype_size(src_dtype)) return fail(SEGSCAN_EINVAL, "src is misaligned");
   }
   if (dst_len > 0 && N > 0) {

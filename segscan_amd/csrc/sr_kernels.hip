@@ -37,6 +37,10 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   int32_t* s_r = sr_smem + 2 * p.tpb;                      // [tpb] row / segment
 
   constexpr int U = (G < 16) ? G : 16;  // dense-row loads issued back to back per batch
+  // Full tiles take the predicate-free loop when batches split into fours. (With
+  // 16-byte row slices the fp32 kernel spills a few registers under the 128-VGPR
+  // cap of 1024-lane workgroups and is still ~1 % faster than without the loop.)
+  constexpr bool kFastTiles = U % 4 == 0;
   const int L = threadIdx.x;
   const int g = L / G;
   const int g_lane = L - g * G;
@@ -47,6 +51,13 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   const int nv = min(CF, p.N - col0);
   const bool full = nv == CF;
   const VT* __restrict__ B = static_cast<const VT*>(p.B);
+  const VT* __restrict__ Bc = B + col0;  // only dereferenced by lanes with nv > 0
+  // Dense row k of this lane: row index and leading dimension are both below 2^31,
+  // so one 32 x 32 -> 64-bit multiply-add forms the address.
+  const uint32_t ldb32 = static_cast<uint32_t>(p.ldb);
+  auto row_at = [&](int64_t k) { return Bc + static_cast<uint64_t>(static_cast<uint32_t>(k)) * ldb32; };
+  // Every lane of the wave holds CF columns: full tiles can skip per-lane column checks.
+  const bool lanes_full = __ballot(!full) == 0;
   const int sbase = g * G;  // this group's LDS slots
   const bool acc_mode = p.accumulate != 0;
   const bool owner = p.policy == 2;
@@ -128,10 +139,24 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
     };
     prefetch(0);
 
-    const int ntile = static_cast<int>((p.chunk + G - 1) / G);  // wave-uniform
-    for (int j = 0; j < ntile; ++j) {
-      const int64_t off = static_cast<int64_t>(j) * G;
-      if (__ballot(off < clen) == 0) break;  // no group of this wave has entries left
+    // Entry (row r, value v) whose dense row is in b: a new row flushes the previous one.
+    auto consume = [&](int r, float v, const float (&b)[CF]) {
+      if (r != cur) {
+        flush(cur, false);
+        if (owner && !acc_mode && nv > 0) {
+          for (int z = cur + 1; z < r; ++z) zero_cols<CF>(p.C + static_cast<int64_t>(z) * p.ldc + col0, nv);
+        }
+        cur = r;
+#pragma unroll
+        for (int i = 0; i < CF; ++i) acc[i] = 0.0f;
+      }
+#pragma unroll
+      for (int i = 0; i < CF; ++i) acc[i] += v * b[i];
+    };
+
+    // Stages the tile at `off`: rows found, (column, value, row) written to the
+    // group's LDS slots. Returns the group's entry count in the tile.
+    auto stage = [&](int64_t off) -> int {
       const bool valid = off + g_lane < clen;
       const int cnt = static_cast<int>(max<int64_t>(0, min<int64_t>(G, clen - off)));
       int row;
@@ -150,7 +175,52 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      return cnt;
+    };
 
+    // A tile that is full for every group of the wave, in a wave whose lanes all
+    // hold CF columns: no per-entry predicates, and the staged metadata is read
+    // four entries per LDS access.
+    auto fast_tile = [&](int64_t off, int cnt) {
+      if constexpr (kFastTiles) {
+        for (int pp = 0; pp < cnt; pp += U) {
+          RawCols<VT, CF> raw[U];
+#pragma unroll
+          for (int u = 0; u < U; u += 4) {
+            int4 kk;
+            if constexpr (SEG) {
+              const int e0 = static_cast<int>(c0 + off + pp + u);
+              kk = make_int4(e0, e0 + 1, e0 + 2, e0 + 3);
+            } else {
+              kk = *reinterpret_cast<const int4*>(s_k + sbase + pp + u);
+            }
+            load_raw<VT, CF>(row_at(kk.x), raw[u]);
+            load_raw<VT, CF>(row_at(kk.y), raw[u + 1]);
+            load_raw<VT, CF>(row_at(kk.z), raw[u + 2]);
+            load_raw<VT, CF>(row_at(kk.w), raw[u + 3]);
+          }
+          if (pp == 0) prefetch(off + G);
+#pragma unroll
+          for (int u = 0; u < U; u += 4) {
+            const int4 rr = *reinterpret_cast<const int4*>(s_r + sbase + pp + u);
+            float4 vv = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            if constexpr (!SEG) vv = *reinterpret_cast<const float4*>(s_v + sbase + pp + u);
+            float b[CF];
+            widen<VT, CF>(raw[u], b);
+            consume(rr.x, vv.x, b);
+            widen<VT, CF>(raw[u + 1], b);
+            consume(rr.y, vv.y, b);
+            widen<VT, CF>(raw[u + 2], b);
+            consume(rr.z, vv.z, b);
+            widen<VT, CF>(raw[u + 3], b);
+            consume(rr.w, vv.w, b);
+          }
+        }
+      }
+    };
+
+    // Any tile: per-entry predicates for partial tiles and partial column vectors.
+    auto slow_tile = [&](int64_t off, int cnt) {
       for (int pp = 0; pp < cnt; pp += U) {
         // Issue the whole batch of dense-row loads before consuming any; the
         // row and value of each entry are read back from LDS only when consumed.
@@ -159,7 +229,7 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
         for (int u = 0; u < U; ++u) {
           if (full && pp + u < cnt) {
             const int64_t krow = SEG ? (c0 + off + pp + u) : static_cast<int64_t>(s_k[sbase + pp + u]);
-            load_raw<VT, CF>(B + krow * p.ldb + col0, raw[u]);
+            load_raw<VT, CF>(row_at(krow), raw[u]);
           }
         }
         // The next tile's metadata travels while this batch's rows arrive.
@@ -169,30 +239,55 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
           if (pp + u < cnt) {
             const int r = s_r[sbase + pp + u];
             const float v = SEG ? 1.0f : s_v[sbase + pp + u];
-            if (r != cur) {
-              flush(cur, false);
-              if (owner && !acc_mode && nv > 0) {
-                for (int z = cur + 1; z < r; ++z) zero_cols<CF>(p.C + static_cast<int64_t>(z) * p.ldc + col0, nv);
-              }
-              cur = r;
-#pragma unroll
-              for (int i = 0; i < CF; ++i) acc[i] = 0.0f;
-            }
             float b[CF];
             if (full) {
               widen<VT, CF>(raw[u], b);
             } else if (nv > 0) {
               const int64_t krow = SEG ? (c0 + off + pp + u) : static_cast<int64_t>(s_k[sbase + pp + u]);
-              load_cols_partial<VT, CF>(B + krow * p.ldb + col0, nv, b);
+              load_cols_partial<VT, CF>(row_at(krow), nv, b);
             } else {
 #pragma unroll
               for (int i = 0; i < CF; ++i) b[i] = 0.0f;
             }
-#pragma unroll
-            for (int i = 0; i < CF; ++i) acc[i] += v * b[i];
+            consume(r, v, b);
           }
         }
       }
+    };
+
+    // Full tiles run in the fast loop; the first tile that is not full for every
+    // group (only the matrix's last chunk has one) hands over, already staged, to
+    // the predicated loop. The two loops are separate regions for the register
+    // allocator. Tile counts and exits are wave-uniform.
+    const int ntile = static_cast<int>((p.chunk + G - 1) / G);
+    int j = 0, cnt = 0;
+    bool staged = false;
+    if constexpr (kFastTiles) {
+      if (lanes_full) {
+        for (; j < ntile; ++j) {
+          const int64_t off = static_cast<int64_t>(j) * G;
+          if (__ballot(off < clen) == 0) {  // no group of this wave has entries left
+            j = ntile;
+            break;
+          }
+          cnt = stage(off);
+          if (__ballot(cnt != 0 && cnt != G) != 0) {
+            staged = true;
+            break;
+          }
+          fast_tile(off, cnt);
+          __builtin_amdgcn_wave_barrier();  // LDS slots are rewritten by the next tile
+        }
+      }
+    }
+    for (; j < ntile; ++j) {
+      const int64_t off = static_cast<int64_t>(j) * G;
+      if (!staged) {
+        if (__ballot(off < clen) == 0) break;
+        cnt = stage(off);
+      }
+      staged = false;
+      slow_tile(off, cnt);
       __builtin_amdgcn_wave_barrier();  // LDS slots are rewritten by the next tile
     }
     if (chunk_valid && clen > 0) {

@@ -74,6 +74,7 @@ def _spmm(lib, desc, N=8, B=0x1000, ldb=8, C=0x2000, ldc=8, variant=_abi.SPMM_EB
     (dict(desc=_desc(), C=None), _abi.EINVAL),
     (dict(desc=_desc(), ldb=4), _abi.EINVAL),
     (dict(desc=_desc(), ldc=7), _abi.EINVAL),
+    (dict(desc=_desc(), ldb=2**31), _abi.EUNSUPPORTED),  # row offsets are 32-bit row x 32-bit stride
     (dict(desc=_desc(), C=0x2002), _abi.EINVAL),
     (dict(desc=_desc(), dtype=7), _abi.EINVAL),
     (dict(desc=_desc(), flags=4), _abi.EINVAL),
@@ -205,8 +206,10 @@ def test_describe_auto_is_concrete(lib):
     (16384, 524288, 60, 16, _abi.SPMM_RBPR, 16, None),   # mean 32, N 16 -> still a group per row
     (16384, 65536, 14, 16, _abi.SPMM_RBSR, None, None),
     (16384, 65536, 14, 64, _abi.SPMM_RBSR, None, None),
-    # skewed or unknown: element-balanced, thread_nz = nnz / 2^18 within [64 / group, 32]
+    # skewed or unknown: element-balanced, thread_nz = 2^floor(log2(1.125 nnz / 2^18)) within [64 / group, 32]
     (16384, 65536, 0, 64, _abi.SPMM_EBSR, 16, 4),
+    (262144, 4194295, 20000, 64, _abi.SPMM_EBSR, 16, 16),   # just under 2^22: still 16
+    (262144, 2097013, 20000, 64, _abi.SPMM_EBSR, 16, 8),
     (65536, 588341, 65536, 64, _abi.SPMM_EBSR, 16, 4),
     (262144, 4194304, 20000, 64, _abi.SPMM_EBSR, 16, 16),
     (262144, 4194304, 20000, 128, _abi.SPMM_EBSR, 32, 16),
