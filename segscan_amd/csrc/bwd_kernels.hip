@@ -192,7 +192,10 @@ __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
           for (int u = 0; u < U; ++u) {
             float t[1] = {part[u]};
             group_reduce_to_first<G, 1>(t);
-            if (g_lane == 0 && pp + u < cnt) p.dvals[s_p[sbase + pp + u]] = t[0];
+            // non-temporal: a scattered 4-byte store would otherwise hold an L2 line
+            // that the dC gathers reuse (power-law 262144^2, N = 64: 218 -> 196 us with
+            // At-order values); non-temporal value loads measured slower and are not used
+            if (g_lane == 0 && pp + u < cnt) __builtin_nontemporal_store(t[0], p.dvals + s_p[sbase + pp + u]);
           }
         }
       }
