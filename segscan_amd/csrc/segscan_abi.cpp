@@ -190,10 +190,12 @@ segscan_status_t plan_spmm(int64_t M, int64_t nnz, int64_t max_row, int64_t N, s
     //    one 256-byte slice of the dense row per group of lanes (row-balanced
     //    sequential reduction);
     //  * otherwise (skewed, or the longest row unknown): element-balanced
-    //    sequential reduction, <= 4 columns per lane and a group of >= 16
-    //    lanes (or N) over a dense row, owner stores, chunks of >= 64 nonzeros
-    //    sized so that ~16 waves per CU each take about one chunk
-    //    (thread_nz = nnz / 2^18, within [64 / group, 32]).
+    //    sequential reduction with 4-byte row slices per lane (one fp32 column,
+    //    two 16-bit columns when N >= 64), wider only when N needs more than 64
+    //    lanes, so one group of up to 64 lanes covers a dense row; owner
+    //    stores, chunks of >= 64 nonzeros with thread_nz the largest power of
+    //    two <= 1.125 nnz / 2^18 (within [64 / group, 32]), and up to 2048
+    //    workgroups of 256 lanes (32 waves per CU).
     const double mean = M > 0 ? static_cast<double>(nnz) / static_cast<double>(M) : 0.0;
     const bool even = max_row > 0 && static_cast<double>(max_row) <= 4.0 * mean + 32.0;
     int cf = std::min(vec, divisor_cf(N, vec));
@@ -207,8 +209,8 @@ segscan_status_t plan_spmm(int64_t M, int64_t nnz, int64_t max_row, int64_t N, s
       k = mk(-1, 6, 256, 1, 1, cf, 1, 0);
     } else {
       v = SEGSCAN_SPMM_EBSR;
-      cf = std::min(cf, 4);
-      while (cf > 1 && N / cf < std::min<int64_t>(16, N)) cf >>= 1;
+      cf = (bt != SEGSCAN_F32 && N >= 64) ? 2 : 1;
+      while (cf < 8 && ceil_div(N, cf) > 64) cf <<= 1;
       int lanes = static_cast<int>(std::min<int64_t>(64, ceil_div(N, cf)));
       int g = 1;
       while (g < lanes) g <<= 1;
@@ -221,7 +223,7 @@ segscan_status_t plan_spmm(int64_t M, int64_t nnz, int64_t max_row, int64_t N, s
       const int64_t chunk = static_cast<int64_t>(tnz) * g;
       const int64_t gpb = 256 / g;
       const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(ceil_div(nnz, chunk), gpb),
-                                                                     256 * 16 * 64 / 256));
+                                                                     256 * 32 * 64 / 256));
       k = mk(gf == 0 ? -1 : gf, 5, 256, 1, 1, cf, tnz, chunk >= 64 ? SEGSCAN_WRITE_OWNER : SEGSCAN_WRITE_LESSATOM);
       // workers(M) = M * bn / bd = blocks
       k.bn = static_cast<int>(blocks);

@@ -206,16 +206,17 @@ def test_describe_auto_is_concrete(lib):
     (16384, 524288, 60, 16, _abi.SPMM_RBPR, 16, None),   # mean 32, N 16 -> still a group per row
     (16384, 65536, 14, 16, _abi.SPMM_RBSR, None, None),
     (16384, 65536, 14, 64, _abi.SPMM_RBSR, None, None),
-    # skewed or unknown: element-balanced, thread_nz = 2^floor(log2(1.125 nnz / 2^18)) within [64 / group, 32]
-    (16384, 65536, 0, 64, _abi.SPMM_EBSR, 16, 4),
-    (262144, 4194295, 20000, 64, _abi.SPMM_EBSR, 16, 16),   # just under 2^22: still 16
-    (262144, 2097013, 20000, 64, _abi.SPMM_EBSR, 16, 8),
-    (65536, 588341, 65536, 64, _abi.SPMM_EBSR, 16, 4),
-    (262144, 4194304, 20000, 64, _abi.SPMM_EBSR, 16, 16),
-    (262144, 4194304, 20000, 128, _abi.SPMM_EBSR, 32, 16),
+    # skewed or unknown: element-balanced, 4-byte slices per lane (one fp32 column) in groups of
+    # up to 64 lanes, thread_nz = 2^floor(log2(1.125 nnz / 2^18)) within [64 / group, 32]
+    (16384, 65536, 0, 64, _abi.SPMM_EBSR, 64, 1),
+    (262144, 4194295, 20000, 64, _abi.SPMM_EBSR, 64, 16),   # just under 2^22: still 16
+    (262144, 2097013, 20000, 64, _abi.SPMM_EBSR, 64, 8),
+    (65536, 588341, 65536, 64, _abi.SPMM_EBSR, 64, 2),
+    (262144, 4194304, 20000, 64, _abi.SPMM_EBSR, 64, 16),
+    (262144, 4194304, 20000, 128, _abi.SPMM_EBSR, 64, 16),  # two columns per lane
     (65536, 1048576, 20000, 16, _abi.SPMM_EBSR, 16, 4),
     (65536, 1048576, 20000, 4, _abi.SPMM_EBSR, 4, 16),
-    (262144, 2**28, 20000, 64, _abi.SPMM_EBSR, 16, 32),
+    (262144, 2**28, 20000, 64, _abi.SPMM_EBSR, 64, 32),
 ])
 def test_describe_auto_follows_row_hint(lib, nrow, nnz, max_row, N, variant, group, tnz):
     info = _describe(lib, nrow, nnz, N, _abi.SPMM_AUTO, max_row=max_row)
@@ -224,13 +225,16 @@ def test_describe_auto_follows_row_hint(lib, nrow, nnz, max_row, N, variant, gro
         assert info.group_size == group
     if tnz is not None:
         assert info.thread_nz == tnz
-    if variant == _abi.SPMM_EBSR:  # about 16 waves per CU on 256 CUs, owner stores
-        assert info.grid_x <= 1024 and info.write_policy == _abi.WRITE_OWNER and info.chunk >= 64
+    if variant == _abi.SPMM_EBSR:  # up to 32 waves per CU on 256 CUs, owner stores
+        assert info.grid_x <= 2048 and info.write_policy == _abi.WRITE_OWNER and info.chunk >= 64
 
 
-def test_describe_auto_bf16_keeps_four_columns_per_lane(lib):
-    info = _describe(lib, 262144, 4194304, 128, _abi.SPMM_AUTO, dtype=_abi.BF16, max_row=20000)
-    assert (info.variant, info.group_size, info.coarsen_factor) == (_abi.SPMM_EBSR, 32, 4)
+def test_describe_auto_16bit_takes_two_columns_per_lane(lib):
+    for N, group in ((64, 32), (128, 64)):
+        info = _describe(lib, 262144, 4194304, N, _abi.SPMM_AUTO, dtype=_abi.BF16, max_row=20000)
+        assert (info.variant, info.group_size, info.coarsen_factor) == (_abi.SPMM_EBSR, group, 2)
+    info = _describe(lib, 262144, 4194304, 512, _abi.SPMM_AUTO, dtype=_abi.F32, max_row=20000)
+    assert (info.group_size, info.coarsen_factor) == (64, 8)  # widened only when N needs more than 64 lanes
 
 
 def test_workspace_size_query(lib):

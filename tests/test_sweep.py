@@ -136,7 +136,8 @@ def test_tiny_sweep_runs_and_checks(gpu, tmp_path):
 
 
 def _bucket_rows():
-    times = {"d1": {"a1": 1.0, "a2": 2.0}, "d2": {"a1": 1.0, "a2": 1.5}, "d3": {"a1": 2.5, "a2": 1.0}}
+    times = {"d1": {"a1": 1.0, "a2": 2.0}, "d2": {"a1": 1.0, "a2": 1.5}, "d3": {"a1": 2.5, "a2": 1.0},
+             "d4": {"a1": 1.0, "a2": 1.2}}
     params = {"a1": {"group_factor": 4, "thread_per_block": 256, "coarsen_factor": 4, "thread_nz": 8,
                      "waves_per_cu": 16, "write_policy": 2},
               "a2": {"group_factor": 3, "tile_factor": 3, "thread_per_block": 256, "block_numer": 1,
@@ -157,20 +158,24 @@ def test_fit_table_and_leave_one_out():
     table = autotune.fit_table(rows)
     key = "gfx950|spmm|f32|N64|skew0|s"
     assert list(table) == [key]
-    e = table[key]  # gmeans: a1 2.5^(1/3) = 1.357, a2 3^(1/3) = 1.442
+    e = table[key]  # gmeans: a1 2.5^(1/4) = 1.257, a2 3.6^(1/4) = 1.377
     assert e["action"] == "a1" and e["variant"] == "ebsr" and e["knobs"]["waves_per_cu"] == 16
-    assert e["gmean_vs_best"] == pytest.approx(2.5 ** (1 / 3), abs=1e-4) and e["matrices"] == ["d1", "d2", "d3"]
+    assert e["gmean_vs_best"] == pytest.approx(2.5 ** (1 / 4), abs=1e-4)
+    assert e["matrices"] == ["d1", "d2", "d3", "d4"]
+    # without d1: a1 2.5^(1/3) = 1.357, a2 1.8^(1/3) = 1.216
     assert autotune.fit_table(rows, exclude="d1")[key]["action"] == "a2"
-    # leave-one-out picks: d1 -> a2 (2.0), d2 -> a2 (1.5), d3 -> a1 (2.5)
+    # leave-one-out picks: d1 -> a2 (2.0), d2 -> a2 (1.5; 2.4^(1/3) = 1.339 < 1.357),
+    # d3 -> a1 (2.5), d4 -> a1 (1.0)
     loo = sweep.analyse(rows)["N64_f32"]["table_loo_vs_best"]
-    assert loo["mean"] == pytest.approx(2.0) and loo["max"] == pytest.approx(2.5) and loo["argmax"] == "d3"
-    # an entry must beat the built-in "auto" by the margin, and rest on >= 2 matrices
+    assert loo["mean"] == pytest.approx(1.75) and loo["max"] == pytest.approx(2.5) and loo["argmax"] == "d3"
+    # an entry must beat the built-in "auto" by the margin, and rest on >= 3 matrices
     with_auto = list(rows)
-    for d, t in (("d1", 1.0), ("d2", 1.0), ("d3", 1.1)):  # auto gmean 1.1^(1/3) = 1.032 < 1.357
+    for d, t in (("d1", 1.0), ("d2", 1.0), ("d3", 1.1), ("d4", 1.0)):  # auto gmean 1.1^(1/4) = 1.024 < 1.257
         with_auto.append({**[r for r in rows if r["dataset"] == d][0], "action": "auto", "family": "select",
                           "kind": "auto", "ms": t})
     assert autotune.fit_table(with_auto) == {}
-    assert autotune.fit_table(rows, exclude="d1", min_matrices=3) == {}
+    assert autotune.fit_table(rows, exclude="d1", min_matrices=4) == {}
+    assert autotune.fit_table([r for r in rows if r["dataset"] in ("d1", "d2")]) == {}  # two matrices: no entry
 
 
 def test_lookup_prefers_matrix_entry_and_resolves_grid():
