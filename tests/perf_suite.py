@@ -42,19 +42,12 @@ def _emit(out, rec):
 
 
 def _time(fn, reps=20, warmup=3):
+    """Median device milliseconds of fn() (HIP events; a spin kernel queued before the
+    start event hides the host's issue cost, as in the autotuner)."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    ts = []
-    for _ in range(reps):
-        s = torch.cuda.Event(enable_timing=True)
-        e = torch.cuda.Event(enable_timing=True)
-        s.record()
-        fn()
-        e.record()
-        e.synchronize()
-        ts.append(s.elapsed_time(e))
-    return float(np.median(ts))
+    return autotune.time_call(fn, reps)
 
 
 def _b_alg(m, N, s_b):
