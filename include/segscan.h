@@ -75,11 +75,11 @@ typedef enum segscan_spmm_variant {
   SEGSCAN_SPMM_AUTO = 8
 } segscan_spmm_variant_t;
 
-/* segment_coo variants (new-cg/include/segscan/segscan.cuh). AUTO runs a
- * dedicated sequential kernel of this library (owner stores, 256-row chunks, the
- * next rows in flight while the current ones are summed); describe() reports it
- * as SEGSCAN_SEG_LESSATOM_SR, the reference variant with the same result.
- * Under AUTO, thread_nz and block_numer / block_denom may set its chunk and grid. */
+/* segment_coo variants (new-cg/include/segscan/segscan.cuh). AUTO runs the
+ * sequential kernel of SEGSCAN_SEG_LESSATOM_SR (which describe() reports) with
+ * owner stores and a gfx950 shape: two columns per lane, ~64 KB of src per chunk,
+ * 8 (4-byte src) or 16 (2-byte src) waves per CU. Under AUTO, thread_nz and
+ * block_numer / block_denom may set its chunk and grid. */
 typedef enum segscan_seg_variant {
   SEGSCAN_SEG_PR = 0,               /* segment_coo */
   SEGSCAN_SEG_PR_NEW = 1,           /* segment_coo_new */

@@ -140,9 +140,6 @@ hipError_t launch_spmm_sr(VType vt, int cf, int g, const SrParams& p, const Laun
 hipError_t launch_spmm_rbsr(VType vt, int cf, const RbsrParams& p, const Launch& l, bool& ok);
 hipError_t launch_seg_pr(VType vt, int cf, int g, const PrParams& p, const Launch& l, bool& ok);
 hipError_t launch_seg_sr(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok);
-// segment_coo "auto": dedicated sequential kernel (owner policy, columns per lane <= 4,
-// 256-lane workgroups, p.total_groups = groups in the grid).
-hipError_t launch_seg_seq(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok);
 hipError_t launch_sddmm_sr(VType vt, int cf, int g, const SddmmParams& p, const Launch& l, bool& ok);
 // Fused backward (bwd_kernels.hip): l.shmem = 12 * p.tpb, p.tpb <= 256.
 hipError_t launch_spmm_bwd(VType vt, int cf, int g, const BwdParams& p, const Launch& l, bool& ok);

@@ -419,7 +419,6 @@ VType vtype(segscan_dtype_t t) { return static_cast<VType>(static_cast<int>(t));
 struct SegPlan {
   int variant = 0;
   bool pr = true;
-  bool seq = false;  // the dedicated sequential segment kernel ("auto")
   int G = 1, T = 1, CF = 1, tnz = 1, policy = SEGSCAN_WRITE_ATOMIC;
   int X = 1, P = 1, Y = 1, slice = 1, block = 64, tpb = 64;
   int64_t grid_x = 1, grid_y = 1, ntiles = 0, chunk = 0, nchunks = 0, total_groups = 0;
@@ -434,25 +433,40 @@ segscan_status_t plan_seg(int64_t E, int64_t N, segscan_dtype_t st, int variant,
   Knobs k = user;
   int v = variant;
   if (v == SEGSCAN_SEG_AUTO) {
-    // Streaming src: a group of lanes per src row, at most 4 columns per lane
-    // (16-bit src: 16 lanes x 64-bit beat 8 lanes x 128-bit, 156 vs 235 us at
-    // N = 64), owner chunks of 256 rows, one chunk per group (the tuned point on
-    // the E = 4.19M, N = 64 power-law index).
+    // Streaming src through the sequential kernel's full-tile loop: a group of
+    // lanes per src row with two columns per lane (wider only when N needs more
+    // than 64 lanes), owner chunks of ~64 KB of src (256 fp32 / 512 bf16 rows of
+    // 64 columns), 8 waves per CU for 4-byte src and 16 for 2-byte src. On the
+    // E = 4.19M, N = 64 power-law index: fp32 235 us, bf16 136 us, against 253 /
+    // 150 us for the previous dedicated kernel.
     v = SEGSCAN_SEG_LESSATOM_SR;
-    int cf = std::min(std::min(vec, 4), divisor_cf(N, vec));
+    int cf = std::min(2, divisor_cf(N, vec));
+    while (cf < 8 && ceil_div(N, cf) > 64) cf <<= 1;
     int lanes = static_cast<int>(std::min<int64_t>(64, ceil_div(N, cf)));
     int g = 1;
     while (g < lanes) g <<= 1;
     int gf = 0;
     while ((1 << gf) < g) ++gf;
-    k = mk(gf == 0 ? -1 : gf, 5, 256, 1, 1, cf, std::max(1, 256 / g), SEGSCAN_WRITE_OWNER);
+    const int64_t row_bytes = std::max<int64_t>(1, N * static_cast<int64_t>(dtype_size(st)));
+    const int64_t gpb = 256 / g;
+    const int64_t waves_per_cu = dtype_size(st) == 4 ? 8 : 16;
+    // ~64 KB of src per chunk, but short enough that every group of the grid gets one
+    const int64_t groups = 256 * waves_per_cu * 64 / g;
+    int64_t rows = std::min<int64_t>(1024, std::max<int64_t>(64, 65536 / row_bytes));
+    rows = std::max<int64_t>(g, std::min<int64_t>(rows, std::max<int64_t>(64, ceil_div(E, groups))));
+    const int tnz = static_cast<int>(std::max<int64_t>(1, rows / g));
+    const int64_t chunk = static_cast<int64_t>(tnz) * g;
+    const int64_t blocks = std::max<int64_t>(
+        1, std::min<int64_t>(ceil_div(ceil_div(E, chunk), gpb), 256 * waves_per_cu * 64 / 256));
+    k = mk(gf == 0 ? -1 : gf, 5, 256, 1, 1, cf, tnz, SEGSCAN_WRITE_OWNER);
+    k.bn = 1;
+    k.bd = static_cast<int>(std::max<int64_t>(1, E / blocks));  // workers(E) = E / bd = blocks
     // the chunk length and grid may be given explicitly (thread_nz, block_numer / block_denom)
     if (user.thread_nz > 0) k.thread_nz = user.thread_nz;
     if (user.bd > 0) {
       k.bd = user.bd;
       k.bn = user.bn > 0 ? user.bn : 1;
     }
-    pl.seq = true;
   }
   pl.variant = v;
   switch (v) {
@@ -916,12 +930,8 @@ segscan_status_t segscan_segment_coo(const void* src, int64_t ld_src, segscan_dt
       p.carry_val = reinterpret_cast<float*>(ws + pl.off_cval);
     }
     mark_start(s);
-    if (pl.seq) {
-      e = launch_seg_seq(vt, pl.CF, pl.G, p, l, compiled);
-    } else {
-      l.shmem = static_cast<size_t>(pl.tpb) * 12;
-      e = launch_seg_sr(vt, pl.CF, pl.G, p, l, compiled);
-    }
+    l.shmem = static_cast<size_t>(pl.tpb) * 12;
+    e = launch_seg_sr(vt, pl.CF, pl.G, p, l, compiled);
     mark_stop(s);
     if (e == hipSuccess && compiled && pl.policy == SEGSCAN_WRITE_OWNER)
       e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, dst, ld_dst, s);

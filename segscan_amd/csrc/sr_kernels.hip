@@ -300,163 +300,6 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   }
 }
 
-// segment_coo, dedicated sequential kernel (the "auto" segment variant). In
-// segment_coo the dense row of entry e is e itself, so nothing needs staging:
-// each group of G lanes (G*CF columns of a src row) walks a chunk of
-// consecutive entries in tiles of 8, the tile's segment ids are loaded by the
-// group's lanes and broadcast by shuffles, and the rows come in two halves of
-// 4 loads: the next half is in flight while the current one is consumed. Owner
-// write policy, as the sequential kernel's: the chunk owns segments
-// (index[c0-1], index[c1-1]], zero-fills its empty ones, and sends a first
-// segment that began in an earlier chunk to carry_val for the fix-up pass.
-template <typename VT, int CF, int G>
-__global__ void __launch_bounds__(256) seg_seq_kernel(SrParams p) {
-  // Tiles of 8 entries in two halves of 4: 96 VGPRs, 5 waves per SIMD. (Halves
-  // of 8 need 152 VGPRs, 3 waves per SIMD, and were 13 % slower in fp32.)
-  constexpr int T = 8;      // entries per tile
-  constexpr int H = T / 2;  // rows per half tile (loads in flight while the other half is consumed)
-  constexpr int PER = (T + G - 1) / G;  // segment ids loaded per lane per tile
-  const int L = threadIdx.x;
-  const int g = L / G;
-  const int gl = L - g * G;
-  const int gbase = __lane_id() - gl;
-  const int64_t ggid = static_cast<int64_t>(blockIdx.x) * (blockDim.x / G) + g;
-  const int col0 = blockIdx.y * (G * CF) + gl * CF;
-  const int nv = min(CF, p.N - col0);
-  const bool full = nv == CF;
-  const VT* __restrict__ src = static_cast<const VT*>(p.B);
-  const bool acc_mode = p.accumulate != 0;
-  const int ntile = static_cast<int>((p.chunk + T - 1) / T);  // wave-uniform
-
-  for (int64_t c = ggid;; c += p.total_groups) {
-    if (__ballot(c < p.nchunks) == 0) break;
-    const bool chunk_valid = c < p.nchunks;
-    const int64_t c0 = chunk_valid ? c * p.chunk : 0;
-    const int64_t clen = chunk_valid ? min(p.chunk, p.nnz - c0) : 0;
-    const int first_seg = chunk_valid ? p.index[c0] : 0;
-    const int prev_seg = (chunk_valid && c0 > 0) ? p.index[c0 - 1] : -1;
-    const bool is_carry = chunk_valid && c0 > 0 && prev_seg == first_seg;
-    if (chunk_valid) {
-      if (gl == 0 && blockIdx.y == 0) p.carry_row[c] = is_carry ? first_seg : -1;
-      if (!acc_mode && nv > 0)
-        for (int z = prev_seg + 1; z < first_seg; ++z) zero_cols<CF>(p.C + static_cast<int64_t>(z) * p.ldc + col0, nv);
-    }
-    int cur = first_seg;
-    float acc[CF];
-#pragma unroll
-    for (int q = 0; q < CF; ++q) acc[q] = 0.0f;
-    auto flush = [&](int seg) {
-      if (nv <= 0) return;
-      if (seg == first_seg && is_carry) store_cols<CF>(p.carry_val + c * p.N + col0, nv, acc, false);
-      else store_cols<CF>(p.C + static_cast<int64_t>(seg) * p.ldc + col0, nv, acc, acc_mode);
-    };
-    auto load_ids = [&](int64_t t0, int (&ids)[PER]) {
-#pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const int i = gl + k * G;
-        ids[k] = (i < T && t0 + i < clen) ? p.index[c0 + t0 + i] : 0;
-      }
-    };
-    auto load_rows = [&](int64_t r0, RawCols<VT, CF> (&raw)[H]) {
-#pragma unroll
-      for (int u = 0; u < H; ++u)
-        if (full && r0 + u < clen) load_raw<VT, CF>(src + (c0 + r0 + u) * p.ldb + col0, raw[u]);
-    };
-    // consume half `hc` (0 or 1, a compile-time constant) of the tile starting at t0
-    auto consume = [&](auto hc, int64_t t0, const RawCols<VT, CF> (&raw)[H], const int (&ids)[PER]) {
-      constexpr int half = decltype(hc)::value;
-#pragma unroll
-      for (int u = 0; u < H; ++u) {
-        const int i = half * H + u;
-        const int sid = __shfl(ids[i / G], gbase + (i % G));
-        const int64_t r = t0 + i;
-        if (r < clen) {
-          if (sid != cur) {
-            flush(cur);
-            if (!acc_mode && nv > 0)
-              for (int z = cur + 1; z < sid; ++z) zero_cols<CF>(p.C + static_cast<int64_t>(z) * p.ldc + col0, nv);
-            cur = sid;
-#pragma unroll
-            for (int q = 0; q < CF; ++q) acc[q] = 0.0f;
-          }
-          float b[CF];
-          if (full) widen<VT, CF>(raw[u], b);
-          else if (nv > 0) load_cols_partial<VT, CF>(src + (c0 + r) * p.ldb + col0, nv, b);
-          else {
-#pragma unroll
-            for (int q = 0; q < CF; ++q) b[q] = 0.0f;
-          }
-#pragma unroll
-          for (int q = 0; q < CF; ++q) acc[q] += b[q];
-        }
-      }
-    };
-
-    int ids[PER];
-    load_ids(0, ids);
-    RawCols<VT, CF> lo[H], hi[H];
-    load_rows(0, lo);
-    for (int t = 0; t < ntile; ++t) {
-      const int64_t t0 = static_cast<int64_t>(t) * T;
-      if (__ballot(t0 < clen) == 0) break;  // no group of this wave has entries left
-      load_rows(t0 + H, hi);
-      consume(std::integral_constant<int, 0>{}, t0, lo, ids);
-      int next_ids[PER];
-      load_ids(t0 + T, next_ids);
-      load_rows(t0 + T, lo);
-      consume(std::integral_constant<int, 1>{}, t0, hi, ids);
-#pragma unroll
-      for (int k = 0; k < PER; ++k) ids[k] = next_ids[k];
-    }
-    if (chunk_valid && clen > 0) {
-      flush(cur);
-      if (!acc_mode && nv > 0 && c == p.nchunks - 1)  // the last chunk owns the trailing empty segments
-        for (int z = cur + 1; z < p.M; ++z) zero_cols<CF>(p.C + static_cast<int64_t>(z) * p.ldc + col0, nv);
-    }
-  }
-}
-
-template <typename VT, int CF, int G>
-static hipError_t launch_seg_seq_t(const SrParams& p, const Launch& l) {
-  seg_seq_kernel<VT, CF, G><<<l.grid, l.block, 0, l.stream>>>(p);
-  return hipGetLastError();
-}
-
-template <typename VT, int CF>
-static hipError_t launch_seg_seq_g(int g, const SrParams& p, const Launch& l, bool& ok) {
-  ok = true;
-  switch (g) {
-    case 1: return launch_seg_seq_t<VT, CF, 1>(p, l);
-    case 2: return launch_seg_seq_t<VT, CF, 2>(p, l);
-    case 4: return launch_seg_seq_t<VT, CF, 4>(p, l);
-    case 8: return launch_seg_seq_t<VT, CF, 8>(p, l);
-    case 16: return launch_seg_seq_t<VT, CF, 16>(p, l);
-    case 32: return launch_seg_seq_t<VT, CF, 32>(p, l);
-    case 64: return launch_seg_seq_t<VT, CF, 64>(p, l);
-    default: ok = false; return hipSuccess;
-  }
-}
-
-template <typename VT>
-static hipError_t launch_seg_seq_cf(int cf, int g, const SrParams& p, const Launch& l, bool& ok) {
-  switch (cf) {
-    case 1: return launch_seg_seq_g<VT, 1>(g, p, l, ok);
-    case 2: return launch_seg_seq_g<VT, 2>(g, p, l, ok);
-    case 4: return launch_seg_seq_g<VT, 4>(g, p, l, ok);
-    default: ok = false; return hipSuccess;
-  }
-}
-
-hipError_t launch_seg_seq(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok) {
-  switch (vt) {
-    case VType::F32: return launch_seg_seq_cf<float>(cf, g, p, l, ok);
-    case VType::BF16: return launch_seg_seq_cf<bf16_t>(cf, g, p, l, ok);
-    case VType::F16: return launch_seg_seq_cf<f16_t>(cf, g, p, l, ok);
-  }
-  ok = false;
-  return hipSuccess;
-}
-
 // SDDMM on A's pattern with the same sequential walk: out[p] = <X[row(p), :],
 // Y[colind[p], :]>. A group of G lanes covers G*CF columns (looping over wider
 // N), walks a chunk of consecutive nnz, finds rows through its own rowptr
