@@ -17,6 +17,10 @@ namespace segscan {
 
 template <typename VT, int CF>
 __global__ void __launch_bounds__(1024) rbsr_kernel(RbsrParams p) {
+  // Dense-row loads per batch. 4: banded 120000^2 N = 64 fp32 57 -> 43 us against
+  // the two-at-a-time loop; 8 and 16 spill under the 128-VGPR cap of 1024-lane
+  // workgroups and lose.
+  constexpr int RB = 4;
   const int L = threadIdx.x;
   const int yr = L / p.tpr;
   const int xv = L - yr * p.tpr;
@@ -24,6 +28,7 @@ __global__ void __launch_bounds__(1024) rbsr_kernel(RbsrParams p) {
   const int col0 = blockIdx.y * p.slice + xv * CF;
   const int nv = min(CF, p.N - col0);
   if (nv <= 0) return;
+  const bool full = nv == CF;
   const VT* __restrict__ B = static_cast<const VT*>(p.B);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * p.rows_per_block;
   for (int64_t r = static_cast<int64_t>(blockIdx.x) * p.rows_per_block + yr; r < p.rows_total; r += stride) {
@@ -33,7 +38,30 @@ __global__ void __launch_bounds__(1024) rbsr_kernel(RbsrParams p) {
 #pragma unroll
     for (int i = 0; i < CF; ++i) acc[i] = 0.0f;
     int q = start;
-    // Two nnz per step so that two dense-row loads are in flight.
+    // Batches of RB nnz: RB dense-row loads in flight before any is consumed.
+    for (; q + RB <= end; q += RB) {
+      int k[RB];
+      float v[RB];
+#pragma unroll
+      for (int u = 0; u < RB; ++u) {
+        k[u] = p.colind[q + u];
+        v[u] = p.vals ? p.vals[q + u] : 1.0f;
+      }
+      RawCols<VT, CF> raw[RB];
+      if (full) {
+#pragma unroll
+        for (int u = 0; u < RB; ++u) load_raw<VT, CF>(B + static_cast<int64_t>(k[u]) * p.ldb + col0, raw[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < RB; ++u) {
+        float b[CF];
+        if (full) widen<VT, CF>(raw[u], b);
+        else load_cols_partial<VT, CF>(B + static_cast<int64_t>(k[u]) * p.ldb + col0, nv, b);
+#pragma unroll
+        for (int i = 0; i < CF; ++i) acc[i] += v[u] * b[i];
+      }
+    }
+    // The rest two at a time.
     for (; q + 1 < end; q += 2) {
       const int64_t k0 = p.colind[q];
       const int64_t k1 = p.colind[q + 1];

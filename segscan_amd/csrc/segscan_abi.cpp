@@ -197,10 +197,7 @@ segscan_status_t plan_spmm(int64_t M, int64_t nnz, int64_t max_row, int64_t N, s
     //    two <= 1.125 nnz / 2^18 (within [64 / group, 32]), and up to 2048
     //    workgroups of 256 lanes (32 waves per CU).
     const double mean = M > 0 ? static_cast<double>(nnz) / static_cast<double>(M) : 0.0;
-    // fp32 at 32 < N <= 64 with >= 2^19 nonzeros: the element-balanced shape below beats
-    // rbsr on even rows too (sweep at N = 64: 6 matrices, geomean 0.91x; N = 128 does not gain)
-    const bool even = max_row > 0 && static_cast<double>(max_row) <= 4.0 * mean + 32.0 &&
-                      !(bt == SEGSCAN_F32 && N > 32 && N <= 64 && nnz >= (int64_t{1} << 19));
+    const bool even = max_row > 0 && static_cast<double>(max_row) <= 4.0 * mean + 32.0;
     int cf = std::min(vec, divisor_cf(N, vec));
     if (even && (N <= 8 || (N <= 16 && mean >= 24.0))) {
       v = SEGSCAN_SPMM_RBPR;

@@ -207,8 +207,7 @@ def test_describe_auto_is_concrete(lib):
     (16384, 65536, 14, 16, _abi.SPMM_RBSR, None, None),
     (16384, 65536, 14, 64, _abi.SPMM_RBSR, None, None),
     (16384, 1048576, 90, 128, _abi.SPMM_RBSR, None, None),
-    # fp32, 32 < N <= 64, >= 2^19 nnz: element-balanced even on even rows
-    (16384, 1048576, 90, 64, _abi.SPMM_EBSR, 64, 4),
+    (16384, 1048576, 90, 64, _abi.SPMM_RBSR, None, None),
     # skewed or unknown: element-balanced, 4-byte slices per lane (one fp32 column) in groups of
     # up to 64 lanes, thread_nz = 2^floor(log2(1.125 nnz / 2^18)) within [64 / group, 32]
     (16384, 65536, 0, 64, _abi.SPMM_EBSR, 64, 1),
