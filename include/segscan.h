@@ -218,9 +218,9 @@ segscan_status_t segscan_sddmm_csr(const segscan_csr_t* A, int64_t N,
  * At.colind = rowind, At.nrow = K, At.ncol = M. Values: At.vals when non-NULL (At
  * order, e.g. the transpose's vals_t: read coalesced; `vals` is then ignored), else
  * vals in A's (CSR) order read through perm (a random 4-byte read per entry), else
- * all ones. dC: M x N (ld_dc), B: K x N (ld_b),
- * both of `dtype`; dB (K x N, ld_db) and dvals (nnz) are fp32, and either may be
- * NULL (not computed). One group of lanes covers all N columns, so N <= 512 (fp32:
+ * all ones. dC: M x N (ld_dc) of dc_dtype, B: K x N (ld_b) of b_dtype: the same
+ * type, or an fp32 dC with a 16-bit B (the operand kept in bf16/fp16 storage);
+ * dB (K x N, ld_db) and dvals (nnz) are fp32, and either may be NULL (not computed). One group of lanes covers all N columns, so N <= 512 (fp32:
  * N <= 256 unless N is a multiple of 8); wider N returns SEGSCAN_EUNSUPPORTED (use
  * segscan_spmm_csr on At plus segscan_sddmm_csr). Knobs: group_factor,
  * coarsen_factor, thread_nz, thread_per_block (<= 256), block_numer / block_denom
@@ -228,11 +228,12 @@ segscan_status_t segscan_sddmm_csr(const segscan_csr_t* A, int64_t N,
  * flags: SEGSCAN_ACCUMULATE adds into dB; dvals is always overwritten. Workspace
  * (only when dB != NULL): segscan_spmm_backward_workspace_size. No counterpart in
  * the reference, whose kernels are forward only. */
-segscan_status_t segscan_spmm_backward_workspace_size(const segscan_csr_t* At, int64_t N, segscan_dtype_t dtype,
+segscan_status_t segscan_spmm_backward_workspace_size(const segscan_csr_t* At, int64_t N, segscan_dtype_t dc_dtype,
                                                       const segscan_knobs_t* knobs, size_t* bytes);
 segscan_status_t segscan_spmm_backward_csr(const segscan_csr_t* At, const int32_t* perm, const float* vals,
-                                           int64_t N, const void* dC, int64_t ld_dc, const void* B, int64_t ld_b,
-                                           segscan_dtype_t dtype, float* dB, int64_t ld_db, float* dvals,
+                                           int64_t N, const void* dC, int64_t ld_dc, segscan_dtype_t dc_dtype,
+                                           const void* B, int64_t ld_b, segscan_dtype_t b_dtype,
+                                           float* dB, int64_t ld_db, float* dvals,
                                            const segscan_knobs_t* knobs, uint32_t flags,
                                            void* workspace, size_t workspace_bytes, void* stream);
 /* Gather (backward of segment_coo): dsrc[e, :] = ddst[index[e], :] * scale(index[e]),

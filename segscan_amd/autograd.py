@@ -84,7 +84,11 @@ class SpmmFunction(torch.autograd.Function):
             # learnable values: both gradients from one walk over the transpose, the values
             # read through the permutation inside the kernel
             v32 = vals.detach().float().contiguous()
-            dB, dvals = ops.spmm_backward(pattern.T, v32, dC, B.detach().float().contiguous(), want_dB=want_b)
+            # a 16-bit B is read as stored (fp32 dC, fp32 accumulation)
+            Bs = B.detach().contiguous()
+            if Bs.dtype not in (torch.bfloat16, torch.float16):
+                Bs = Bs.float()
+            dB, dvals = ops.spmm_backward(pattern.T, v32, dC, Bs, want_dB=want_b)
             return dvals.to(vals.dtype), (None if dB is None else dB.to(B.dtype)), None
         # dB alone (fixed values): the SpMM kernels on the transpose, values in its order (cached)
         if ctx.needs_input_grad[1]:

@@ -85,12 +85,13 @@ class BatchedGcnStep:
         ops.segment_gather(dP, self.b.graph_index, out=self.dHt)
         if dH is not None:
             self.dHt.add_(dH)
-        Xf = X if X.dtype == torch.float32 else X.float()
         if self.fused:
-            # the batch's values are fixed: read them in the transpose's order (cached at set-up)
-            ops.spmm_backward(self.T, None, self.dHt, Xf, dB=self.dX, dvals=self.dvals, workspace=self.ws_b,
+            # the batch's values are fixed: read them in the transpose's order (cached at set-up);
+            # a 16-bit X is read as stored
+            ops.spmm_backward(self.T, None, self.dHt, X, dB=self.dX, dvals=self.dvals, workspace=self.ws_b,
                               vals_t=self.At.vals)
             return
+        Xf = X if X.dtype == torch.float32 else X.float()
         ops.spmm(self.At, self.dHt, self.variant_t, self.knobs_t, out=self.dX, workspace=self.ws_t)
         ops.sddmm(A, self.dHt, Xf, out=self.dvals)
 

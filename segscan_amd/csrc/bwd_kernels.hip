@@ -27,7 +27,8 @@
 
 namespace segscan {
 
-template <typename VT, int CF, int G>
+// XT: storage type of dC (gathered); YT: storage type of B (one row per At row).
+template <typename XT, typename YT, int CF, int G>
 __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
   extern __shared__ int32_t bwd_smem[];
   int32_t* s_k = bwd_smem;               // [tpb] dC row of the entry (a row of A)
@@ -44,8 +45,8 @@ __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
   const int col0 = g_lane * CF;
   const int nv = min(CF, p.N - col0);
   const bool full = nv == CF;
-  const VT* __restrict__ X = static_cast<const VT*>(p.dC);  // gathered by At's column index
-  const VT* __restrict__ Y = static_cast<const VT*>(p.B);   // row j of At
+  const XT* __restrict__ X = static_cast<const XT*>(p.dC);  // gathered by At's column index
+  const YT* __restrict__ Y = static_cast<const YT*>(p.B);   // row j of At
   const int sbase = g * G;
   const bool want_db = p.dB != nullptr;
   const bool want_dv = p.dvals != nullptr;
@@ -87,12 +88,12 @@ __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
     float y[CF];
 #pragma unroll
     for (int i = 0; i < CF; ++i) y[i] = 0.0f;
-    RawCols<VT, CF> ynx;
+    RawCols<YT, CF> ynx;
     auto load_y = [&](int r) {
-      if (nv > 0) load_cols_any<VT, CF>(Y + static_cast<int64_t>(r) * p.ld_b + col0, nv, y);
+      if (nv > 0) load_cols_any<YT, CF>(Y + static_cast<int64_t>(r) * p.ld_b + col0, nv, y);
     };
     auto issue_next = [&](int r) {
-      if (full && r + 1 < p.K) load_raw<VT, CF>(Y + static_cast<int64_t>(r + 1) * p.ld_b + col0, ynx);
+      if (full && r + 1 < p.K) load_raw<YT, CF>(Y + static_cast<int64_t>(r + 1) * p.ld_b + col0, ynx);
     };
     if (want_dv && chunk_valid && clen > 0) {
       load_y(first_row);
@@ -143,10 +144,10 @@ __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
       for (int pp = 0; pp < cnt; pp += U) {
-        RawCols<VT, CF> xr[U];
+        RawCols<XT, CF> xr[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          if (full && pp + u < cnt) load_raw<VT, CF>(X + static_cast<int64_t>(s_k[sbase + pp + u]) * p.ld_dc + col0, xr[u]);
+          if (full && pp + u < cnt) load_raw<XT, CF>(X + static_cast<int64_t>(s_k[sbase + pp + u]) * p.ld_dc + col0, xr[u]);
         }
         if (pp == 0) prefetch(off + G);
         float part[U];
@@ -163,7 +164,7 @@ __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
                 for (int z = cur + 1; z < r; ++z) zero_cols<CF>(p.dB + static_cast<int64_t>(z) * p.ld_db + col0, nv);
               }
               if (want_dv) {
-                if (full && r == cur + 1) widen<VT, CF>(ynx, y);
+                if (full && r == cur + 1) widen<YT, CF>(ynx, y);
                 else load_y(r);
                 issue_next(r);
               }
@@ -173,9 +174,9 @@ __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
             }
             float xa[CF];
             if (full) {
-              widen<VT, CF>(xr[u], xa);
+              widen<XT, CF>(xr[u], xa);
             } else if (nv > 0) {
-              load_cols_partial<VT, CF>(X + static_cast<int64_t>(s_k[sbase + pp + u]) * p.ld_dc + col0, nv, xa);
+              load_cols_partial<XT, CF>(X + static_cast<int64_t>(s_k[sbase + pp + u]) * p.ld_dc + col0, nv, xa);
             } else {
 #pragma unroll
               for (int i = 0; i < CF; ++i) xa[i] = 0.0f;
@@ -211,43 +212,50 @@ __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
   }
 }
 
-template <typename VT, int CF, int G>
+template <typename XT, typename YT, int CF, int G>
 static hipError_t launch_bwd_t(const BwdParams& p, const Launch& l) {
-  bwd_sr_kernel<VT, CF, G><<<l.grid, l.block, l.shmem, l.stream>>>(p);
+  bwd_sr_kernel<XT, YT, CF, G><<<l.grid, l.block, l.shmem, l.stream>>>(p);
   return hipGetLastError();
 }
 
-template <typename VT, int CF>
+template <typename XT, typename YT, int CF>
 static hipError_t launch_bwd_g(int g, const BwdParams& p, const Launch& l, bool& ok) {
   ok = true;
   switch (g) {
-    case 1: return launch_bwd_t<VT, CF, 1>(p, l);
-    case 2: return launch_bwd_t<VT, CF, 2>(p, l);
-    case 4: return launch_bwd_t<VT, CF, 4>(p, l);
-    case 8: return launch_bwd_t<VT, CF, 8>(p, l);
-    case 16: return launch_bwd_t<VT, CF, 16>(p, l);
-    case 32: return launch_bwd_t<VT, CF, 32>(p, l);
-    case 64: return launch_bwd_t<VT, CF, 64>(p, l);
+    case 1: return launch_bwd_t<XT, YT, CF, 1>(p, l);
+    case 2: return launch_bwd_t<XT, YT, CF, 2>(p, l);
+    case 4: return launch_bwd_t<XT, YT, CF, 4>(p, l);
+    case 8: return launch_bwd_t<XT, YT, CF, 8>(p, l);
+    case 16: return launch_bwd_t<XT, YT, CF, 16>(p, l);
+    case 32: return launch_bwd_t<XT, YT, CF, 32>(p, l);
+    case 64: return launch_bwd_t<XT, YT, CF, 64>(p, l);
     default: ok = false; return hipSuccess;
   }
 }
 
-template <typename VT>
+template <typename XT, typename YT>
 static hipError_t launch_bwd_cf(int cf, int g, const BwdParams& p, const Launch& l, bool& ok) {
   switch (cf) {
-    case 1: return launch_bwd_g<VT, 1>(g, p, l, ok);
-    case 2: return launch_bwd_g<VT, 2>(g, p, l, ok);
-    case 4: return launch_bwd_g<VT, 4>(g, p, l, ok);
-    case 8: return launch_bwd_g<VT, 8>(g, p, l, ok);
+    case 1: return launch_bwd_g<XT, YT, 1>(g, p, l, ok);
+    case 2: return launch_bwd_g<XT, YT, 2>(g, p, l, ok);
+    case 4: return launch_bwd_g<XT, YT, 4>(g, p, l, ok);
+    case 8: return launch_bwd_g<XT, YT, 8>(g, p, l, ok);
     default: ok = false; return hipSuccess;
   }
 }
 
-hipError_t launch_spmm_bwd(VType vt, int cf, int g, const BwdParams& p, const Launch& l, bool& ok) {
-  switch (vt) {
-    case VType::F32: return launch_bwd_cf<float>(cf, g, p, l, ok);
-    case VType::BF16: return launch_bwd_cf<bf16_t>(cf, g, p, l, ok);
-    case VType::F16: return launch_bwd_cf<f16_t>(cf, g, p, l, ok);
+// dC and B share a storage type, or dC is fp32 and B 16-bit (the gradient kept in
+// fp32 against a 16-bit stored operand).
+hipError_t launch_spmm_bwd(VType xt, VType yt, int cf, int g, const BwdParams& p, const Launch& l, bool& ok) {
+  if (xt == yt) {
+    switch (xt) {
+      case VType::F32: return launch_bwd_cf<float, float>(cf, g, p, l, ok);
+      case VType::BF16: return launch_bwd_cf<bf16_t, bf16_t>(cf, g, p, l, ok);
+      case VType::F16: return launch_bwd_cf<f16_t, f16_t>(cf, g, p, l, ok);
+    }
+  } else if (xt == VType::F32) {
+    if (yt == VType::BF16) return launch_bwd_cf<float, bf16_t>(cf, g, p, l, ok);
+    if (yt == VType::F16) return launch_bwd_cf<float, f16_t>(cf, g, p, l, ok);
   }
   ok = false;
   return hipSuccess;

@@ -442,26 +442,51 @@ hipError_t launch_sddmm(VType vt, const int32_t* rowptr, const int32_t* colind, 
 
 // ---- segment gather / counts ----
 
+// dsrc[e, :] = ddst[index[e], :] * scale. Each lane keeps one column position of
+// the row it is on (vpr lanes per row, rows_per_block rows per workgroup pass) and
+// walks rows grid-strided: no per-element division, and 16-byte loads and stores
+// when N and the leading dimensions allow (V = 4).
+template <int V>
 __global__ void segment_gather_kernel(const float* __restrict__ ddst, int64_t ld_ddst, const int32_t* __restrict__ index,
                                       int64_t E, int N, const int32_t* __restrict__ counts, float* __restrict__ dsrc,
-                                      int64_t ld_dsrc) {
-  const int64_t n = E * N;
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int64_t e = i / N;
-    const int64_t j = i - e * N;
+                                      int64_t ld_dsrc, int vpr, int rows_per_block) {
+  const int t = threadIdx.x;
+  const int ry = t / vpr;
+  const int col = (t - ry * vpr) * V;
+  if (ry >= rows_per_block) return;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * rows_per_block;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * rows_per_block + ry; e < E; e += stride) {
     const int s = index[e];
-    float scale = 1.0f;
-    if (counts) scale = 1.0f / static_cast<float>(max(counts[s], 1));
-    dsrc[e * ld_dsrc + j] = ddst[static_cast<int64_t>(s) * ld_ddst + j] * scale;
+    const float scale = counts ? 1.0f / static_cast<float>(max(counts[s], 1)) : 1.0f;
+    const float* src = ddst + static_cast<int64_t>(s) * ld_ddst;
+    float* dst = dsrc + e * ld_dsrc;
+    if constexpr (V == 4) {
+      float4 x = *reinterpret_cast<const float4*>(src + col);
+      x.x *= scale;
+      x.y *= scale;
+      x.z *= scale;
+      x.w *= scale;
+      *reinterpret_cast<float4*>(dst + col) = x;
+    } else {
+      for (int c = col; c < N; c += vpr) dst[c] = src[c] * scale;
+    }
   }
 }
 
 hipError_t launch_segment_gather(const float* ddst, int64_t ld_ddst, const int32_t* index, int64_t E, int N,
                                  const int32_t* counts, float* dsrc, int64_t ld_dsrc, hipStream_t s) {
   if (E <= 0 || N <= 0) return hipSuccess;
-  segment_gather_kernel<<<grid_for(E * N, kBlock, 65536), kBlock, 0, s>>>(ddst, ld_ddst, index, E, N, counts, dsrc,
-                                                                         ld_dsrc);
+  const bool vec = N % 4 == 0 && N <= 4 * kBlock && ld_ddst % 4 == 0 && ld_dsrc % 4 == 0 &&
+                   reinterpret_cast<uintptr_t>(ddst) % 16 == 0 && reinterpret_cast<uintptr_t>(dsrc) % 16 == 0;
+  const int vpr = vec ? N / 4 : (N < kBlock ? N : kBlock);
+  const int rows_per_block = kBlock / vpr;
+  const dim3 grid = grid_for(E, rows_per_block, 65536);
+  if (vec)
+    segment_gather_kernel<4><<<grid, kBlock, 0, s>>>(ddst, ld_ddst, index, E, N, counts, dsrc, ld_dsrc, vpr,
+                                                     rows_per_block);
+  else
+    segment_gather_kernel<1><<<grid, kBlock, 0, s>>>(ddst, ld_ddst, index, E, N, counts, dsrc, ld_dsrc, vpr,
+                                                     rows_per_block);
   return hipGetLastError();
 }
 

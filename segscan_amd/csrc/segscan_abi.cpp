@@ -1062,8 +1062,9 @@ segscan_status_t plan_bwd(int64_t K, int64_t nnz, int64_t N, segscan_dtype_t dt,
   return SEGSCAN_OK;
 }
 
-segscan_status_t segscan_spmm_backward_workspace_size(const segscan_csr_t* At, int64_t N, segscan_dtype_t dtype,
+segscan_status_t segscan_spmm_backward_workspace_size(const segscan_csr_t* At, int64_t N, segscan_dtype_t dc_dtype,
                                                       const segscan_knobs_t* knobs, size_t* bytes) {
+  const segscan_dtype_t dtype = dc_dtype;
   if (!bytes) return fail(SEGSCAN_EINVAL, "bytes is NULL");
   segscan_status_t st = validate_csr(At, false);
   if (st) return st;
@@ -1077,17 +1078,20 @@ segscan_status_t segscan_spmm_backward_workspace_size(const segscan_csr_t* At, i
 }
 
 segscan_status_t segscan_spmm_backward_csr(const segscan_csr_t* At, const int32_t* perm, const float* vals, int64_t N,
-                                           const void* dC, int64_t ld_dc, const void* B, int64_t ld_b,
-                                           segscan_dtype_t dtype, float* dB, int64_t ld_db, float* dvals,
-                                           const segscan_knobs_t* knobs, uint32_t flags, void* workspace,
-                                           size_t workspace_bytes, void* stream) {
+                                           const void* dC, int64_t ld_dc, segscan_dtype_t dc_dtype, const void* B,
+                                           int64_t ld_b, segscan_dtype_t b_dtype, float* dB, int64_t ld_db,
+                                           float* dvals, const segscan_knobs_t* knobs, uint32_t flags,
+                                           void* workspace, size_t workspace_bytes, void* stream) {
   segscan_status_t st = validate_csr(At, true);
   if (st) return st;
   if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N = %lld out of range", (long long)N);
-  if (!dtype_valid(dtype)) return fail(SEGSCAN_EINVAL, "unknown dtype %d", (int)dtype);
+  if (!dtype_valid(dc_dtype) || !dtype_valid(b_dtype))
+    return fail(SEGSCAN_EINVAL, "unknown dtype %d / %d", (int)dc_dtype, (int)b_dtype);
+  if (dc_dtype != b_dtype && dc_dtype != SEGSCAN_F32)
+    return fail(SEGSCAN_EUNSUPPORTED, "dC and B must share a dtype, or dC be fp32");
   if (flags & ~SEGSCAN_ACCUMULATE) return fail(SEGSCAN_EINVAL, "unknown flags 0x%x", flags);
   const int64_t K = At->nrow, nnz = At->nnz;
-  const size_t es = dtype_size(dtype);
+  const size_t es = dtype_size(dc_dtype), es_b = dtype_size(b_dtype);
   if (dB && K > 0 && N > 0) {
     if (ld_db < N) return fail(SEGSCAN_EINVAL, "ld_db %lld < N %lld", (long long)ld_db, (long long)N);
     if (reinterpret_cast<uintptr_t>(dB) % 4) return fail(SEGSCAN_EINVAL, "dB is not 4-byte aligned");
@@ -1097,16 +1101,16 @@ segscan_status_t segscan_spmm_backward_csr(const segscan_csr_t* At, const int32_
     if (N > 0) {
       if (!dC || !B) return fail(SEGSCAN_EINVAL, "dC / B is NULL");
       if (ld_dc < N || ld_b < N) return fail(SEGSCAN_EINVAL, "leading dimension < N");
-      if (reinterpret_cast<uintptr_t>(dC) % es || reinterpret_cast<uintptr_t>(B) % es)
+      if (reinterpret_cast<uintptr_t>(dC) % es || reinterpret_cast<uintptr_t>(B) % es_b)
         return fail(SEGSCAN_EINVAL, "dC / B is misaligned");
     }
   }
   const bool accumulate = flags & SEGSCAN_ACCUMULATE;
   const int64_t ldd = dB ? std::max<int64_t>(ld_db, 1) : std::max<int64_t>(ld_dc, 1);
   const int cap = std::min(aligned_cf(8, dC, std::max<int64_t>(ld_dc, 1), es, dB, ldd),
-                           aligned_cf(8, B, std::max<int64_t>(ld_b, 1), es, nullptr, std::max<int64_t>(ld_b, 1)));
+                           aligned_cf(8, B, std::max<int64_t>(ld_b, 1), es_b, nullptr, std::max<int64_t>(ld_b, 1)));
   BwdPlan pl;
-  st = plan_bwd(K, nnz, std::max<int64_t>(N, 1), dtype, from_c(knobs), cap, pl);
+  st = plan_bwd(K, nnz, std::max<int64_t>(N, 1), dc_dtype, from_c(knobs), cap, pl);
   if (st) return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (N == 0) {
@@ -1154,7 +1158,7 @@ segscan_status_t segscan_spmm_backward_csr(const segscan_csr_t* At, const int32_
   l.stream = s;
   bool compiled = true;
   mark_start(s);
-  hipError_t e = launch_spmm_bwd(vtype(dtype), pl.CF, pl.G, p, l, compiled);
+  hipError_t e = launch_spmm_bwd(vtype(dc_dtype), vtype(b_dtype), pl.CF, pl.G, p, l, compiled);
   mark_stop(s);
   if (!compiled)
     return fail(SEGSCAN_EUNSUPPORTED, "no compiled backward kernel for coarsen %d, group %d", pl.CF, pl.G);

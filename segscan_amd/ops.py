@@ -238,8 +238,8 @@ def spmm_backward(T: CscMatrix, vals: Optional[torch.Tensor], dC: torch.Tensor, 
     and used instead of vals. Returns (dB or None, dvals or None)."""
     _require_device(dC, B)
     lib = _abi.load()
-    if dC.dtype != B.dtype:
-        raise TypeError("dC and B must share a dtype")
+    if dC.dtype != B.dtype and dC.dtype != torch.float32:
+        raise TypeError("dC and B must share a dtype, or dC be float32")
     K, M, nnz = T.t.nrow, T.t.ncol, T.t.nnz
     N = B.size(1)
     if tuple(B.shape) != (K, N) or tuple(dC.shape) != (M, N):
@@ -257,10 +257,10 @@ def spmm_backward(T: CscMatrix, vals: Optional[torch.Tensor], dC: torch.Tensor, 
             raise ValueError(f"{name} must be contiguous float32 with nnz elements")
     kp = _abi.knobs_ptr(knobs)
     if workspace is None:
-        workspace = spmm_backward_workspace(T, N, B.dtype, knobs, B.device) if dB is not None else None
+        workspace = spmm_backward_workspace(T, N, dC.dtype, knobs, B.device) if dB is not None else None
     desc = T.t.with_vals(vals_t).desc()  # the At-order values, if any, travel in At.vals
     st = lib.segscan_spmm_backward_csr(byref(desc), _ptr(T.perm), _ptr(vals), N, _ptr(dC), _leading_dim(dC, "dC"),
-                                       _ptr(B), _leading_dim(B, "B"), _dtype_code(B), _ptr(dB),
+                                       _dtype_code(dC), _ptr(B), _leading_dim(B, "B"), _dtype_code(B), _ptr(dB),
                                        _leading_dim(dB, "dB") if dB is not None else 0, _ptr(dvals), kp,
                                        _abi.ACCUMULATE if accumulate else 0, _ptr(workspace),
                                        0 if workspace is None else workspace.numel(), _stream(stream))

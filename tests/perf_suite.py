@@ -56,9 +56,9 @@ def _b_alg(m, N, s_b):
 
 
 def _best(A, B, stats, reps=5, limit=None):
-    cands = autotune.spmm_candidates(B.size(1), B.dtype, stats)
+    cands = [("auto", None)] + autotune.spmm_candidates(B.size(1), B.dtype, stats)
     if limit:
-        cands = cands[:8] + [c for c in cands if c[0] == "ebsr"][:limit]
+        cands = cands[:9] + [c for c in cands if c[0] == "ebsr"][:limit]
     res = autotune.tune_spmm(A, B, cands, reps=reps)
     return res[0]["variant"], res[0]["knobs"], res
 

@@ -127,20 +127,26 @@ def test_backward_validation(lib):
 def test_fused_backward_validation_and_plan(lib):
     d = _desc(nrow=10, ncol=12, nnz=20)
     f = lib.segscan_spmm_backward_csr
-    #     At       perm  vals  N  dC    ld  B     ld dtype dB    ld  dvals knobs fl  ws    bytes stream
-    assert f(byref(d), None, None, 8, 0x100, 8, 0x200, 8, 0, 0x300, 8, 0x400, None, 0, 0x500, 1 << 20, None) == _abi.EINVAL
-    assert f(byref(d), 0x10, None, 8, None, 8, 0x200, 8, 0, 0x300, 8, 0x400, None, 0, 0x500, 1 << 20, None) == _abi.EINVAL
-    assert f(byref(d), 0x10, None, 8, 0x100, 4, 0x200, 8, 0, 0x300, 8, 0x400, None, 0, 0x500, 1 << 20, None) == _abi.EINVAL
-    assert f(byref(d), 0x10, None, 8, 0x100, 8, 0x200, 8, 0, 0x300, 4, 0x400, None, 0, 0x500, 1 << 20, None) == _abi.EINVAL
-    assert f(byref(d), 0x10, None, 8, 0x100, 8, 0x200, 8, 0, 0x300, 8, 0x400, None, 2, 0x500, 1 << 20, None) == _abi.EINVAL
-    assert f(byref(d), 0x10, None, 8, 0x100, 8, 0x200, 8, 0, 0x300, 8, 0x400, None, 0, None, 0, None) == _abi.EWORKSPACE
+
+    def call(perm=0x10, N=8, dC=0x100, ld_dc=8, dc_t=_abi.F32, B=0x200, ld_b=8, b_t=_abi.F32, dB=0x300, ld_db=8,
+             dvals=0x400, knobs=None, flags=0, ws=0x500, ws_bytes=1 << 30):
+        return f(byref(d), perm, None, N, dC, ld_dc, dc_t, B, ld_b, b_t, dB, ld_db, dvals, knobs, flags, ws, ws_bytes,
+                 None)
+
+    assert call(perm=None) == _abi.EINVAL
+    assert call(dC=None) == _abi.EINVAL
+    assert call(ld_dc=4) == _abi.EINVAL
+    assert call(ld_db=4) == _abi.EINVAL
+    assert call(flags=2) == _abi.EINVAL
+    assert call(ws=None, ws_bytes=0) == _abi.EWORKSPACE
+    assert call(dc_t=9) == _abi.EINVAL
+    # a 16-bit dC needs a B of the same type; an fp32 dC takes any B
+    assert call(dc_t=_abi.BF16, b_t=_abi.F32) == _abi.EUNSUPPORTED
+    assert call(dc_t=_abi.BF16, b_t=_abi.F16) == _abi.EUNSUPPORTED
     # wider than one group of 64 lanes x 8 columns
-    assert f(byref(d), 0x10, None, 600, 0x100, 600, 0x200, 600, 0, 0x300, 600, 0x400, None, 0, 0x500, 1 << 30,
-             None) == _abi.EUNSUPPORTED
+    assert call(N=600, ld_dc=600, ld_b=600, ld_db=600) == _abi.EUNSUPPORTED
     assert "coarsen_factor" in lib.segscan_last_error().decode()
-    kk = _abi.Knobs.make(thread_per_block=512)
-    assert f(byref(d), 0x10, None, 8, 0x100, 8, 0x200, 8, 0, 0x300, 8, 0x400, byref(kk), 0, 0x500, 1 << 20,
-             None) == _abi.EUNSUPPORTED
+    assert call(knobs=byref(_abi.Knobs.make(thread_per_block=512))) == _abi.EUNSUPPORTED
     # workspace: one carry row index and one N-wide carry per chunk
     n = c_size_t(0)
     big = _desc(nrow=262144, ncol=262144, nnz=4194304)

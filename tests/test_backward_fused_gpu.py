@@ -95,6 +95,25 @@ def test_fused_backward_16bit_operands(gpu, oracle, dtype):
     assert oracle.normalized_error(dv.cpu().numpy(), rv, rv_abs) <= 1e-6
 
 
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
+def test_fused_backward_fp32_grad_16bit_operand(gpu, oracle, dtype):
+    """fp32 dC with B kept in 16-bit storage (read as stored): against the oracle on
+    the rounded B, and bit-equal to the fused kernel on an fp32 copy of the rounded B."""
+    m = MATS["symmetric_powerlaw"]
+    N = 64
+    rng = np.random.default_rng(21)
+    dC = torch.from_numpy(rng.standard_normal((m.nrow, N)).astype(np.float32)).to(gpu)
+    B16 = torch.from_numpy(rng.standard_normal((m.ncol, N)).astype(np.float32)).to(dtype).to(gpu)
+    A = m.to_device(gpu)
+    T = ops.csr_transpose(A)
+    dB, dv = ops.spmm_backward(T, A.vals, dC, B16)
+    rB, rB_abs, rv, rv_abs = _oracle_grads(oracle, m, dC.cpu().numpy(), B16.float().cpu().numpy())
+    assert oracle.normalized_error(dB.cpu().numpy(), rB, rB_abs) <= 1e-6
+    assert oracle.normalized_error(dv.cpu().numpy(), rv, rv_abs) <= 1e-6
+    _, dv32 = ops.spmm_backward(T, A.vals, dC, B16.float(), want_dB=False)
+    assert torch.equal(dv, dv32)  # same products and order: widening is exact
+
+
 def test_fused_backward_optional_outputs_and_accumulate(gpu, oracle):
     m = MATS["symmetric_powerlaw"]
     N = 32

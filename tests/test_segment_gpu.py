@@ -121,12 +121,13 @@ def test_check_index_flags(gpu):
     assert ops.check_index(torch.tensor([0, 5], dtype=torch.int32, device=gpu), 4) & 2
 
 
-def test_segment_counts_and_gather(gpu):
+@pytest.mark.parametrize("N", [24, 7, 1030])  # 16-byte lanes; one float per lane; columns looped per lane
+def test_segment_counts_and_gather(gpu, N):
     rng = np.random.default_rng(9)
     idx, dst_len = _indices()[1]
     counts = ops.segment_counts(torch.from_numpy(idx).to(gpu), dst_len).cpu().numpy()
     assert np.array_equal(counts, np.bincount(idx, minlength=dst_len))
-    ddst = rng.standard_normal((dst_len, 24)).astype(np.float32)
+    ddst = rng.standard_normal((dst_len, N)).astype(np.float32)
     g = ops.segment_gather(torch.from_numpy(ddst).to(gpu), torch.from_numpy(idx).to(gpu)).cpu().numpy()
     assert np.array_equal(g, ddst[idx])
     gm = ops.segment_gather(torch.from_numpy(ddst).to(gpu), torch.from_numpy(idx).to(gpu),
