@@ -100,7 +100,7 @@ def spmm_leg(name, m, N, dtype, dev, oracle, out, reps=20, rng=None, extra=None,
     return rec
 
 
-def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "batched")):
+def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "shard", "batched")):
     from oracle import oracle
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(0)
@@ -206,6 +206,32 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "batc
                         "err_dX": oracle.normalized_error(Xg.grad.float().cpu().numpy(), dX64, dXabs),
                         "err_dvals": oracle.normalized_error(vals.grad.cpu().numpy(), dv64, dvabs)})
 
+    if "shard" in legs:
+        # One GPU's share of an 8-way row-sharded SpMM: nnz-balanced row blocks of the
+        # power-law matrix with full K (B replicated), each timed on its own; the
+        # slowest block is what every GPU of an 8-GPU job would wait for.
+        m = synth.powerlaw_csr(seed=0)
+        N = 64
+        B32 = rng.uniform(-1, 1, size=(m.ncol, N)).astype(np.float32)
+        cum = m.rowptr.astype(np.int64)
+        bounds = [0] + [int(np.searchsorted(cum, k * m.nnz // 8)) for k in range(1, 8)] + [m.nrow]
+        for dtype in (torch.float32, torch.bfloat16):
+            B = torch.from_numpy(B32).to(dtype).to(dev)
+            times, nnzs = [], []
+            for k in range(8):
+                r0, r1 = bounds[k], bounds[k + 1]
+                rp = (m.rowptr[r0:r1 + 1] - m.rowptr[r0]).astype(np.int32)
+                sl = slice(int(m.rowptr[r0]), int(m.rowptr[r1]))
+                A = ops.CsrMatrix.from_numpy(r1 - r0, m.ncol, rp, m.colind[sl], m.vals[sl], device=dev)
+                C = torch.empty((A.nrow, N), device=dev)
+                ws = ops.spmm_workspace(A, N, dtype, "auto")
+                times.append(_time(lambda: ops.spmm(A, B, "auto", out=C, workspace=ws), 20))
+                nnzs.append(A.nnz)
+            _emit(out, {"leg": f"shard_1of8_powerlaw_262144_N64_{str(dtype).split('.')[-1]}", "shards": 8,
+                        "rows": [bounds[k + 1] - bounds[k] for k in range(8)], "nnz": nnzs,
+                        "ms": [round(t, 5) for t in times], "max_ms": round(max(times), 5),
+                        "mean_ms": round(float(np.mean(times)), 5)})
+
     if "batched" in legs:
         csr, offsets = synth.batched_graphs(seed=0)
         b = batch.make_batch(csr, offsets, dev)
@@ -244,5 +270,5 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "batc
 if __name__ == "__main__":
     out = sys.argv[1] if len(sys.argv) > 1 else None
     legs = tuple(sys.argv[2].split(",")) if len(sys.argv) > 2 else ("poisson", "powerlaw", "segment", "banded", "gcn",
-                                                                    "batched")
+                                                                    "shard", "batched")
     run(out, legs)
