@@ -21,7 +21,9 @@ INCLUDE = PKG.parent / "include"
 
 SOURCES = [
     "pr_kernels.hip",
-    "sr_kernels.hip",
+    "sr_f32.hip",
+    "sr_bf16.hip",
+    "sr_f16.hip",
     "bwd_kernels.hip",
     "rbsr_kernels.hip",
     "misc_kernels.hip",

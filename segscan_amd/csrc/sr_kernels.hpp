@@ -22,6 +22,8 @@
 //            fix-up pass. No atomics on the main path and no zero-fill pass;
 //  * lessatom: rows that start and end inside the chunk are plain stores, the
 //            chunk's first and last rows atomic adds (output zero-filled first).
+#pragma once
+
 #include <type_traits>
 
 #include "device_common.hpp"
@@ -433,24 +435,6 @@ static hipError_t launch_sr_cf(int cf, int g, const SrParams& p, const Launch& l
   }
 }
 
-static hipError_t launch_sr_vt(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool seg, bool& ok) {
-  switch (vt) {
-    case VType::F32: return launch_sr_cf<float>(cf, g, p, l, seg, ok);
-    case VType::BF16: return launch_sr_cf<bf16_t>(cf, g, p, l, seg, ok);
-    case VType::F16: return launch_sr_cf<f16_t>(cf, g, p, l, seg, ok);
-  }
-  ok = false;
-  return hipSuccess;
-}
-
-hipError_t launch_spmm_sr(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok) {
-  return launch_sr_vt(vt, cf, g, p, l, false, ok);
-}
-
-hipError_t launch_seg_sr(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok) {
-  return launch_sr_vt(vt, cf, g, p, l, true, ok);
-}
-
 template <typename VT, int CF, int G>
 static hipError_t launch_sddmm_t(const SddmmParams& p, const Launch& l) {
   sddmm_sr_kernel<VT, CF, G><<<l.grid, l.block, 0, l.stream>>>(p);
@@ -485,14 +469,13 @@ static hipError_t launch_sddmm_cf(int cf, int g, const SddmmParams& p, const Lau
   }
 }
 
-hipError_t launch_sddmm_sr(VType vt, int cf, int g, const SddmmParams& p, const Launch& l, bool& ok) {
-  switch (vt) {
-    case VType::F32: return launch_sddmm_cf<float>(cf, g, p, l, ok);
-    case VType::BF16: return launch_sddmm_cf<bf16_t>(cf, g, p, l, ok);
-    case VType::F16: return launch_sddmm_cf<f16_t>(cf, g, p, l, ok);
-  }
-  ok = false;
-  return hipSuccess;
-}
+// Per-type entry points (sr_f32.hip, sr_bf16.hip, sr_f16.hip: one translation unit
+// per storage type, compiled in parallel).
+hipError_t launch_sr_f32(int cf, int g, const SrParams& p, const Launch& l, bool seg, bool& ok);
+hipError_t launch_sr_bf16(int cf, int g, const SrParams& p, const Launch& l, bool seg, bool& ok);
+hipError_t launch_sr_f16(int cf, int g, const SrParams& p, const Launch& l, bool seg, bool& ok);
+hipError_t launch_sddmm_f32(int cf, int g, const SddmmParams& p, const Launch& l, bool& ok);
+hipError_t launch_sddmm_bf16(int cf, int g, const SddmmParams& p, const Launch& l, bool& ok);
+hipError_t launch_sddmm_f16(int cf, int g, const SddmmParams& p, const Launch& l, bool& ok);
 
 }  // namespace segscan
