@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-tuned", action="store_true", help="skip the comparison legs (for profiling)")
     ap.add_argument("--dump", type=str, default=None, help="write tuning results to this JSON file")
+    ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "segment_f32", "segment_bf16"], default=None,
+                    help="run one comparison leg alone (for per-kernel profiles) and print its timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB fp32 device-to-device 5 times (scales HBM byte counters)")
     return ap.parse_args()
@@ -83,10 +85,65 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
 
 
+def only_leg(args):
+    """One leg of the bench alone, `warmup` + `steps` calls, for rocprofv3 summaries of a
+    single kernel configuration: the tuned bf16-B SpMM, the fused backward (full: dB and
+    dvals, values through perm), or segment_coo AUTO over the power-law index (E = 4.19M,
+    N = 64). Prints {"leg", "ms"} (HIP events, median per call)."""
+    import numpy as np
+    import torch
+
+    from segscan_amd import autotune, ops, synth
+    dev = torch.device("cuda:0")
+    N = args.N
+    m = synth.powerlaw_csr(seed=0)
+    stats = autotune.row_stats(m.rowptr)
+    A = m.to_device(dev)
+    rng = np.random.default_rng(0)
+    B32 = torch.from_numpy(rng.uniform(-1, 1, size=(m.ncol, N)).astype(np.float32)).to(dev)
+    if args.only_leg == "bf16":
+        Bbf = B32.to(torch.bfloat16)
+        e = autotune.load_table().get(autotune.matrix_key("spmm", torch.bfloat16, N, stats, m.ncol))
+        v, k = (e["variant"], autotune.resolve_knobs(e.get("knobs"), stats)) if e else ("auto", None)
+        C = torch.empty((m.nrow, N), dtype=torch.float32, device=dev)
+        ws = ops.spmm_workspace(A, N, torch.bfloat16, v, k)
+        fn = lambda: ops.spmm(A, Bbf, variant=v, knobs=k, out=C, workspace=ws)  # noqa: E731
+    elif args.only_leg == "fwd_bwd":
+        dC = torch.from_numpy(rng.standard_normal((m.nrow, N)).astype(np.float32)).to(dev)
+        T = ops.csr_transpose(A)
+        dB = torch.empty_like(B32)
+        dvals = torch.empty(m.nnz, dtype=torch.float32, device=dev)
+        ws = ops.spmm_backward_workspace(T, N, torch.float32)
+        fn = lambda: ops.spmm_backward(T, A.vals, dC, B32, dB=dB, dvals=dvals, workspace=ws)  # noqa: E731
+    else:
+        dt = torch.float32 if args.only_leg == "segment_f32" else torch.bfloat16
+        idx = torch.from_numpy(synth.segment_index_from_degrees(m.degrees())).to(dev)
+        src = torch.from_numpy(rng.uniform(-1, 1, size=(m.nnz, N)).astype(np.float32)).to(dt).to(dev)
+        dst = torch.empty((m.nrow, N), dtype=torch.float32, device=dev)
+        info = ops.segment_describe(m.nnz, N, m.nrow, dt, "auto")
+        ws = torch.empty(max(1, info["workspace_bytes"]), dtype=torch.uint8, device=dev)
+        fn = lambda: ops.segment_coo(src, idx, m.nrow, "auto", out=dst, workspace=ws)  # noqa: E731
+    for _ in range(args.warmup):
+        fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(args.steps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    print(json.dumps({"leg": args.only_leg, "ms": round(float(np.median(ts)), 5)}))
+
+
 def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.only_leg:
+        only_leg(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this single-GPU build "
