@@ -65,7 +65,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-tuned", action="store_true", help="skip the comparison legs (for profiling)")
     ap.add_argument("--dump", type=str, default=None, help="write tuning results to this JSON file")
-    ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "segment_f32", "segment_bf16"], default=None,
+    ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "segment_f32", "segment_bf16", "batched_graph"],
+                    default=None,
                     help="run one comparison leg alone (for per-kernel profiles) and print its timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB fp32 device-to-device 5 times (scales HBM byte counters)")
@@ -88,18 +89,28 @@ def copy_calibration():
 def only_leg(args):
     """One leg of the bench alone, `warmup` + `steps` calls, for rocprofv3 summaries of a
     single kernel configuration: the tuned bf16-B SpMM, the fused backward (full: dB and
-    dvals, values through perm), or segment_coo AUTO over the power-law index (E = 4.19M,
-    N = 64). Prints {"leg", "ms"} (HIP events, median per call)."""
+    dvals, values through perm), segment_coo AUTO over the power-law index (E = 4.19M,
+    N = 64), or one replay of the batched step's hipGraph (256 uneven graphs, 2.39M nodes,
+    9.58M nnz, bf16 features, N = 64: SpMM, per-graph sum pooling, pooling gather and the
+    fused backward). Prints {"leg", "ms"} (HIP events, median per call)."""
     import numpy as np
     import torch
 
-    from segscan_amd import autotune, ops, synth
+    from segscan_amd import autotune, batch, ops, synth
     dev = torch.device("cuda:0")
     N = args.N
+    rng = np.random.default_rng(0)
+    if args.only_leg == "batched_graph":
+        csr, offsets = synth.batched_graphs(seed=0)
+        gb = batch.make_batch(csr, offsets, dev)
+        X = torch.from_numpy(rng.uniform(-1, 1, size=(csr.nrow, N)).astype(np.float32)).to(dev).to(torch.bfloat16)
+        dP = torch.from_numpy(rng.standard_normal((gb.n_graphs, N)).astype(np.float32)).to(dev)
+        graph = batch.BatchedGcnStep(gb, N, dtype=torch.bfloat16).capture(X, dP)
+        _time_leg(args, graph.replay)
+        return
     m = synth.powerlaw_csr(seed=0)
     stats = autotune.row_stats(m.rowptr)
     A = m.to_device(dev)
-    rng = np.random.default_rng(0)
     B32 = torch.from_numpy(rng.uniform(-1, 1, size=(m.ncol, N)).astype(np.float32)).to(dev)
     if args.only_leg == "bf16":
         Bbf = B32.to(torch.bfloat16)
@@ -123,6 +134,13 @@ def only_leg(args):
         info = ops.segment_describe(m.nnz, N, m.nrow, dt, "auto")
         ws = torch.empty(max(1, info["workspace_bytes"]), dtype=torch.uint8, device=dev)
         fn = lambda: ops.segment_coo(src, idx, m.nrow, "auto", out=dst, workspace=ws)  # noqa: E731
+    _time_leg(args, fn)
+
+
+def _time_leg(args, fn):
+    import numpy as np
+    import torch
+
     for _ in range(args.warmup):
         fn()
     torch.cuda.synchronize()

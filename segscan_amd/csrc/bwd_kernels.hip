@@ -21,16 +21,18 @@
 // stores the rows it owns, zero-fills its empty ones, and sends a first row
 // that began in an earlier chunk to carry_val for the fix-up pass). Each dvals
 // entry is written once (perm is a bijection) after a DPP reduction across the
-// group.
+// group. Where At rows are short (a few entries each) the one-row-ahead B load
+// arrives too late; the row-batch form (RB) instead issues the B row of every row
+// that starts inside a batch together with the batch's dC loads.
 #include "device_common.hpp"
 #include "kernel_params.hpp"
 
 namespace segscan {
 
 // XT: storage type of dC (gathered); YT: storage type of B (one row per At row).
-template <typename XT, typename YT, int CF, int G>
-__global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
-  extern __shared__ int32_t bwd_smem[];
+// RB: B rows loaded per batch of entries (short At rows) instead of one row ahead.
+template <typename XT, typename YT, int CF, int G, bool RB>
+__device__ __forceinline__ void bwd_walk(const BwdParams& p, int32_t* bwd_smem) {
   int32_t* s_k = bwd_smem;               // [tpb] dC row of the entry (a row of A)
   int32_t* s_r = bwd_smem + p.tpb;       // [tpb] At row of the entry (a column of A)
   int32_t* s_p = bwd_smem + 2 * p.tpb;   // [tpb] position of the entry in A
@@ -97,7 +99,7 @@ __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
     };
     if (want_dv && chunk_valid && clen > 0) {
       load_y(first_row);
-      issue_next(first_row);
+      if constexpr (!RB) issue_next(first_row);
     }
 
     auto flush = [&](int row) {
@@ -145,9 +147,18 @@ __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
 
       for (int pp = 0; pp < cnt; pp += U) {
         RawCols<XT, CF> xr[U];
+        RawCols<YT, CF> yr[RB ? U : 1];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          if (full && pp + u < cnt) load_raw<XT, CF>(X + static_cast<int64_t>(s_k[sbase + pp + u]) * p.ld_dc + col0, xr[u]);
+          if (full && pp + u < cnt) {
+            load_raw<XT, CF>(X + static_cast<int64_t>(s_k[sbase + pp + u]) * p.ld_dc + col0, xr[u]);
+            if constexpr (RB) {
+              // the B row of an entry that starts a row (the consume loop's test below)
+              const int ru = s_r[sbase + pp + u];
+              const int rp = u == 0 ? cur : s_r[sbase + pp + u - 1];
+              if (want_dv && ru != rp) load_raw<YT, CF>(Y + static_cast<int64_t>(ru) * p.ld_b + col0, yr[u]);
+            }
+          }
         }
         if (pp == 0) prefetch(off + G);
         float part[U];
@@ -164,9 +175,14 @@ __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
                 for (int z = cur + 1; z < r; ++z) zero_cols<CF>(p.dB + static_cast<int64_t>(z) * p.ld_db + col0, nv);
               }
               if (want_dv) {
-                if (full && r == cur + 1) widen<YT, CF>(ynx, y);
-                else load_y(r);
-                issue_next(r);
+                if constexpr (RB) {
+                  if (full) widen<YT, CF>(yr[u], y);
+                  else load_y(r);
+                } else {
+                  if (full && r == cur + 1) widen<YT, CF>(ynx, y);
+                  else load_y(r);
+                  issue_next(r);
+                }
               }
               cur = r;
 #pragma unroll
@@ -213,8 +229,29 @@ __global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
 }
 
 template <typename XT, typename YT, int CF, int G>
+__global__ void __launch_bounds__(256) bwd_sr_kernel(BwdParams p) {
+  extern __shared__ int32_t bwd_smem[];
+  bwd_walk<XT, YT, CF, G, false>(p, bwd_smem);
+}
+
+// Row-batch form, bf16 B only (a batch's 8 B rows take 2 VGPRs each). Held to 4
+// waves per SIMD (at most 128 VGPRs): with an fp32 dC it would otherwise take 130
+// and drop to 3 waves, which costs more than the early B loads gain.
+template <typename XT, typename YT, int CF, int G>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) bwd_sr_rowbatch_kernel(BwdParams p) {
+  extern __shared__ int32_t bwd_smem[];
+  bwd_walk<XT, YT, CF, G, true>(p, bwd_smem);
+}
+
+template <typename XT, typename YT, int CF, int G>
 static hipError_t launch_bwd_t(const BwdParams& p, const Launch& l) {
   bwd_sr_kernel<XT, YT, CF, G><<<l.grid, l.block, l.shmem, l.stream>>>(p);
+  return hipGetLastError();
+}
+
+template <typename XT, int G>
+static hipError_t launch_bwd_rb(const BwdParams& p, const Launch& l) {
+  bwd_sr_rowbatch_kernel<XT, bf16_t, 4, G><<<l.grid, l.block, l.shmem, l.stream>>>(p);
   return hipGetLastError();
 }
 
@@ -246,7 +283,15 @@ static hipError_t launch_bwd_cf(int cf, int g, const BwdParams& p, const Launch&
 
 // dC and B share a storage type, or dC is fp32 and B 16-bit (the gradient kept in
 // fp32 against a 16-bit stored operand).
-hipError_t launch_spmm_bwd(VType xt, VType yt, int cf, int g, const BwdParams& p, const Launch& l, bool& ok) {
+hipError_t launch_spmm_bwd(VType xt, VType yt, int cf, int g, bool row_batch, const BwdParams& p, const Launch& l,
+                           bool& ok) {
+  if (row_batch) {
+    ok = yt == VType::BF16 && (xt == VType::F32 || xt == VType::BF16) && cf == 4 && (g == 8 || g == 16 || g == 32);
+    if (!ok) return hipSuccess;
+    if (xt == VType::F32)
+      return g == 8 ? launch_bwd_rb<float, 8>(p, l) : g == 16 ? launch_bwd_rb<float, 16>(p, l) : launch_bwd_rb<float, 32>(p, l);
+    return g == 8 ? launch_bwd_rb<bf16_t, 8>(p, l) : g == 16 ? launch_bwd_rb<bf16_t, 16>(p, l) : launch_bwd_rb<bf16_t, 32>(p, l);
+  }
   if (xt == yt) {
     switch (xt) {
       case VType::F32: return launch_bwd_cf<float, float>(cf, g, p, l, ok);

@@ -142,8 +142,10 @@ hipError_t launch_seg_pr(VType vt, int cf, int g, const PrParams& p, const Launc
 hipError_t launch_seg_sr(VType vt, int cf, int g, const SrParams& p, const Launch& l, bool& ok);
 hipError_t launch_sddmm_sr(VType vt, int cf, int g, const SddmmParams& p, const Launch& l, bool& ok);
 // Fused backward (bwd_kernels.hip): l.shmem = 12 * p.tpb, p.tpb <= 256. xt: dC storage,
-// yt: B storage (equal, or xt F32 with a 16-bit yt).
-hipError_t launch_spmm_bwd(VType xt, VType yt, int cf, int g, const BwdParams& p, const Launch& l, bool& ok);
+// yt: B storage (equal, or xt F32 with a 16-bit yt). row_batch: the form that loads B
+// rows per batch of entries, compiled for yt BF16, xt F32 or BF16, cf 4, g 8 / 16 / 32.
+hipError_t launch_spmm_bwd(VType xt, VType yt, int cf, int g, bool row_batch, const BwdParams& p, const Launch& l,
+                           bool& ok);
 
 // Helpers (misc_kernels.hip).
 hipError_t launch_fill_zero(float* C, int64_t rows, int64_t N, int64_t ldc, hipStream_t s);

@@ -179,3 +179,50 @@ def test_fused_backward_rejects_too_wide_N(gpu):
     with pytest.raises(_abi.SegscanError) as exc:
         ops.spmm_backward(T, A.vals, torch.ones((m.nrow, N), device=gpu), torch.ones((m.ncol, N), device=gpu))
     assert exc.value.status == _abi.EUNSUPPORTED
+
+
+def _short_row_matrix(seed):
+    """At rows of ~3 entries (nnz < 8 K, the row-batch form's condition), with empty
+    columns of A (empty At rows) between and at the end."""
+    rng = np.random.default_rng(seed)
+    m = synth.small_random_csr(3000, 4000, 0.0008, seed=seed)
+    keep = np.ones(4000, bool)
+    keep[rng.choice(4000, size=600, replace=False)] = False
+    keep[-40:] = False
+    mask = keep[m.colind]
+    deg = np.bincount(m.row_ids()[mask], minlength=m.nrow)
+    rowptr = np.zeros(m.nrow + 1, np.int32)
+    np.cumsum(deg, out=rowptr[1:])
+    colind = m.colind[mask].astype(np.int32)
+    vals = rng.integers(-3, 4, size=colind.size).astype(np.float32)
+    return synth.HostCsr(m.nrow, m.ncol, rowptr, colind, vals)
+
+
+@pytest.mark.parametrize("dc_dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("N,knobs", [
+    (32, None),                                            # 8 lanes x 4 columns
+    (64, None),                                            # 16 x 4
+    (128, None),                                           # 32 x 4
+    (64, {"thread_nz": 1}),                                # carries at nearly every chunk
+    (64, {"block_numer": 1, "block_denom": 1000000}),      # one workgroup, grid-stride over all chunks
+    (61, {"coarsen_factor": 4}),                           # a partial last lane (1 of 4 columns)
+])
+def test_fused_backward_short_rows_bf16_operand(gpu, dc_dtype, N, knobs):
+    """Short At rows with a bf16 B take the row-batch form (B rows loaded with each
+    batch of dC rows); integer operands make every product and sum exact."""
+    m = _short_row_matrix(N + 7)
+    assert m.nnz < 8 * m.ncol and m.nnz > 0
+    rng = np.random.default_rng(N)
+    dC = rng.integers(-3, 4, size=(m.nrow, N)).astype(np.float32)
+    B = rng.integers(-3, 4, size=(m.ncol, N)).astype(np.float32)
+    A = m.to_device(gpu)
+    T = ops.csr_transpose(A)
+    dB = torch.full((m.ncol, N), 5.0, device=gpu)  # every row, empty ones too, must be written
+    _, dv = ops.spmm_backward(T, A.vals, torch.from_numpy(dC).to(gpu).to(dc_dtype),
+                              torch.from_numpy(B).to(gpu).to(torch.bfloat16), dB=dB, knobs=knobs)
+    rows = m.row_ids()
+    want_dB = np.zeros((m.ncol, N))
+    np.add.at(want_dB, m.colind, m.vals[:, None].astype(np.float64) * dC[rows])
+    want_dv = np.einsum("pn,pn->p", dC[rows].astype(np.float64), B[m.colind].astype(np.float64))
+    assert np.array_equal(dB.cpu().numpy(), want_dB.astype(np.float32))
+    assert np.array_equal(dv.cpu().numpy(), want_dv.astype(np.float32))
