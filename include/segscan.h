@@ -78,7 +78,8 @@ typedef enum segscan_spmm_variant {
 /* segment_coo variants (new-cg/include/segscan/segscan.cuh). AUTO runs the
  * sequential kernel of SEGSCAN_SEG_LESSATOM_SR (which describe() reports) with
  * owner stores and a gfx950 shape: two columns per lane, ~64 KB of src per chunk,
- * 8 (4-byte src) or 16 (2-byte src) waves per CU. Under AUTO, thread_nz and
+ * 8 (4-byte src) or 16 (2-byte src) waves per CU, or one chunk per group where
+ * that grid would need at most two rounds of chunks. Under AUTO, thread_nz and
  * block_numer / block_denom may set its chunk and grid. */
 typedef enum segscan_seg_variant {
   SEGSCAN_SEG_PR = 0,               /* segment_coo */
