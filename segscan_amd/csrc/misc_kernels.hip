@@ -445,7 +445,10 @@ hipError_t launch_sddmm(VType vt, const int32_t* rowptr, const int32_t* colind, 
 // dsrc[e, :] = ddst[index[e], :] * scale. Each lane keeps one column position of
 // the row it is on (vpr lanes per row, rows_per_block rows per workgroup pass) and
 // walks rows grid-strided: no per-element division, and 16-byte loads and stores
-// when N and the leading dimensions allow (V = 4).
+// when N and the leading dimensions allow (V = 4). The 16-byte path keeps four rows
+// in flight per lane (four index loads, then four row loads) and writes dsrc, which
+// nothing rereads soon, with non-temporal stores: 2.39M x 64 rows over 256 segments
+// 157 -> 106 us, 4.19M x 64 over 262144 segments 311 -> 193 us.
 template <int V>
 __global__ void segment_gather_kernel(const float* __restrict__ ddst, int64_t ld_ddst, const int32_t* __restrict__ index,
                                       int64_t E, int N, const int32_t* __restrict__ counts, float* __restrict__ dsrc,
@@ -455,19 +458,40 @@ __global__ void segment_gather_kernel(const float* __restrict__ ddst, int64_t ld
   const int col = (t - ry * vpr) * V;
   if (ry >= rows_per_block) return;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * rows_per_block;
-  for (int64_t e = static_cast<int64_t>(blockIdx.x) * rows_per_block + ry; e < E; e += stride) {
-    const int s = index[e];
-    const float scale = counts ? 1.0f / static_cast<float>(max(counts[s], 1)) : 1.0f;
-    const float* src = ddst + static_cast<int64_t>(s) * ld_ddst;
-    float* dst = dsrc + e * ld_dsrc;
-    if constexpr (V == 4) {
-      float4 x = *reinterpret_cast<const float4*>(src + col);
-      x.x *= scale;
-      x.y *= scale;
-      x.z *= scale;
-      x.w *= scale;
-      *reinterpret_cast<float4*>(dst + col) = x;
-    } else {
+  if constexpr (V == 4) {
+    constexpr int R = 4;
+    for (int64_t e0 = static_cast<int64_t>(blockIdx.x) * rows_per_block + ry; e0 < E; e0 += R * stride) {
+      int sr[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const int64_t e = e0 + i * stride;
+        sr[i] = e < E ? index[e] : 0;  // past E: row 0 (E > 0 entries name valid rows), not stored
+      }
+      float4 xr[R];
+      float sc[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        xr[i] = *reinterpret_cast<const float4*>(ddst + static_cast<int64_t>(sr[i]) * ld_ddst + col);
+        sc[i] = counts ? 1.0f / static_cast<float>(max(counts[sr[i]], 1)) : 1.0f;
+      }
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const int64_t e = e0 + i * stride;
+        if (e < E) {
+          float* dst = dsrc + e * ld_dsrc + col;
+          __builtin_nontemporal_store(xr[i].x * sc[i], dst);
+          __builtin_nontemporal_store(xr[i].y * sc[i], dst + 1);
+          __builtin_nontemporal_store(xr[i].z * sc[i], dst + 2);
+          __builtin_nontemporal_store(xr[i].w * sc[i], dst + 3);
+        }
+      }
+    }
+  } else {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * rows_per_block + ry; e < E; e += stride) {
+      const int s = index[e];
+      const float scale = counts ? 1.0f / static_cast<float>(max(counts[s], 1)) : 1.0f;
+      const float* src = ddst + static_cast<int64_t>(s) * ld_ddst;
+      float* dst = dsrc + e * ld_dsrc;
       for (int c = col; c < N; c += vpr) dst[c] = src[c] * scale;
     }
   }
@@ -480,7 +504,7 @@ hipError_t launch_segment_gather(const float* ddst, int64_t ld_ddst, const int32
                    reinterpret_cast<uintptr_t>(ddst) % 16 == 0 && reinterpret_cast<uintptr_t>(dsrc) % 16 == 0;
   const int vpr = vec ? N / 4 : (N < kBlock ? N : kBlock);
   const int rows_per_block = kBlock / vpr;
-  const dim3 grid = grid_for(E, rows_per_block, 65536);
+  const dim3 grid = grid_for(E, vec ? rows_per_block * 4 : rows_per_block, 65536);  // 4 rows per lane pass (V = 4)
   if (vec)
     segment_gather_kernel<4><<<grid, kBlock, 0, s>>>(ddst, ld_ddst, index, E, N, counts, dsrc, ld_dsrc, vpr,
                                                      rows_per_block);
