@@ -221,11 +221,15 @@ segscan_status_t segscan_sddmm_csr(const segscan_csr_t* A, int64_t N,
  * vals in A's (CSR) order read through perm (a random 4-byte read per entry), else
  * all ones. dC: M x N (ld_dc) of dc_dtype, B: K x N (ld_b) of b_dtype: the same
  * type, or an fp32 dC with a 16-bit B (the operand kept in bf16/fp16 storage);
- * dB (K x N, ld_db) and dvals (nnz) are fp32, and either may be NULL (not computed). One group of lanes covers all N columns, so N <= 512 (fp32:
- * N <= 256 unless N is a multiple of 8); wider N returns SEGSCAN_EUNSUPPORTED (use
+ * dB (K x N, ld_db) and dvals (nnz) are fp32, and either may be NULL (not
+ * computed). One group of lanes covers all N columns, so N <= 512 (fp32: N <= 256
+ * unless N is a multiple of 8); wider N returns SEGSCAN_EUNSUPPORTED (use
  * segscan_spmm_csr on At plus segscan_sddmm_csr). Knobs: group_factor,
  * coarsen_factor, thread_nz, thread_per_block (<= 256), block_numer / block_denom
- * (workgroups = K * numer / denom), as for SEGSCAN_SPMM_EBSR; NULL = built-in shape.
+ * (workgroups = K * numer / denom), as for SEGSCAN_SPMM_EBSR; NULL = built-in shape
+ * (chunks of 16 tiles, 8 where At rows average under 8 entries; with such short rows
+ * and a bf16 B, 4 columns per lane and 8-32 lanes, each row's B row is loaded
+ * together with the batch of dC rows that reaches it).
  * flags: SEGSCAN_ACCUMULATE adds into dB; dvals is always overwritten. Workspace
  * (only when dB != NULL): segscan_spmm_backward_workspace_size. No counterpart in
  * the reference, whose kernels are forward only. */
