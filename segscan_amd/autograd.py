@@ -24,13 +24,14 @@ class SpmmPattern:
     """A CSR pattern with its device transpose computed once."""
 
     def __init__(self, A: ops.CsrMatrix, variant="auto", knobs=None, variant_t="auto", knobs_t=None,
-                 fused: bool = True):
+                 fused: bool = True, knobs_bwd=None):
         self.A = A
         T = ops.csr_transpose(A)
         self.T = ops.CscMatrix(T.t.measured(), T.perm)  # longest column guides the "auto" plan of A^T
         self.variant, self.knobs = variant, knobs
         self.variant_t, self.knobs_t = variant_t, knobs_t
         self.fused = fused  # False: always the separate transposed SpMM + SDDMM kernels
+        self.knobs_bwd = knobs_bwd  # fused backward shape (e.g. from autotune.tune_backward); None = built-in
         self._fused = {}
         self._vals_t, self._vals_t_key = None, None
 
@@ -40,7 +41,7 @@ class SpmmPattern:
             return False
         if N not in self._fused:
             try:
-                ops.spmm_backward_workspace(self.T, N)
+                ops.spmm_backward_workspace(self.T, N, knobs=self.knobs_bwd)
                 self._fused[N] = True
             except _abi.SegscanError as exc:
                 if exc.status != _abi.EUNSUPPORTED:
@@ -88,7 +89,7 @@ class SpmmFunction(torch.autograd.Function):
             Bs = B.detach().contiguous()
             if Bs.dtype not in (torch.bfloat16, torch.float16):
                 Bs = Bs.float()
-            dB, dvals = ops.spmm_backward(pattern.T, v32, dC, Bs, want_dB=want_b)
+            dB, dvals = ops.spmm_backward(pattern.T, v32, dC, Bs, want_dB=want_b, knobs=pattern.knobs_bwd)
             return dvals.to(vals.dtype), (None if dB is None else dB.to(B.dtype)), None
         # dB alone (fixed values): the SpMM kernels on the transpose, values in its order (cached)
         if ctx.needs_input_grad[1]:

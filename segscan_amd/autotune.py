@@ -31,7 +31,7 @@ from typing import Iterable
 import numpy as np
 import torch
 
-from . import ops
+from . import _abi, ops
 
 DEFAULT_TABLE = Path(__file__).resolve().parent / "tuned_gfx950.json"
 
@@ -251,6 +251,41 @@ def tune_spmm(A: ops.CsrMatrix, B: torch.Tensor, candidates: Iterable, reps: int
             results.append({"variant": variant, "knobs": knobs, "ms": float("inf"), "error": str(exc)})
             continue
         results.append({"variant": variant, "knobs": knobs, "ms": ms})
+    results.sort(key=lambda r: r["ms"])
+    return results
+
+
+def backward_candidates(N: int) -> list[dict | None]:
+    """Shapes for the fused backward: the built-in one, then 2 / 4 / 8 columns per lane
+    (the group covers all N columns, so at most 64 lanes) with 8- or 16-tile chunks.
+    The best shape depends on the matrix: for example, 8 columns per lane wins on a
+    GCN adjacency at N = 128 but loses on a power-law matrix."""
+    cands: list[dict | None] = [None]
+    for cf in (2, 4, 8):
+        if -(-N // cf) > 64:
+            continue
+        for tnz in (8, 16):
+            cands.append({"coarsen_factor": cf, "thread_nz": tnz})
+    return cands
+
+
+def tune_backward(T: ops.CscMatrix, vals: torch.Tensor | None, dC: torch.Tensor, B: torch.Tensor,
+                  candidates: Iterable, reps: int = 10, vals_t: torch.Tensor | None = None) -> list[dict]:
+    """Times the fused backward (both dB and dvals) over knob points, fastest first;
+    points the library rejects are kept with ms = inf and the error."""
+    N = B.size(1)
+    dB = torch.empty((T.t.nrow, N), dtype=torch.float32, device=B.device)
+    dv = torch.empty(max(T.t.nnz, 1), dtype=torch.float32, device=B.device)[: T.t.nnz]
+    results = []
+    for knobs in candidates:
+        try:
+            ws = ops.spmm_backward_workspace(T, N, dC.dtype, knobs, B.device)
+            ms = time_call(lambda: ops.spmm_backward(T, vals, dC, B, dB=dB, dvals=dv, knobs=knobs, workspace=ws,
+                                                     vals_t=vals_t), reps)
+        except _abi.SegscanError as exc:  # unsupported knob points are skipped, not fatal
+            results.append({"knobs": knobs, "ms": float("inf"), "error": str(exc)})
+            continue
+        results.append({"knobs": knobs, "ms": ms})
     results.sort(key=lambda r: r["ms"])
     return results
 

@@ -178,6 +178,12 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "shar
             ms_sep = _time(fwd_bwd, reps=10)  # transposed SpMM + SDDMM as two kernels
             pattern.fused = True
             ms = _time(fwd_bwd, reps=10)
+            # the fused backward's shape tuned for this matrix (the built-in one is a candidate)
+            Xs = Xg.detach() if dtype != torch.float32 else Xg.detach().float()
+            tb = autotune.tune_backward(pattern.T, vals.detach(), dC, Xs, autotune.backward_candidates(N), reps=5)
+            pattern.knobs_bwd = tb[0]["knobs"]
+            ms_tuned = _time(fwd_bwd, reps=10)
+            pattern.knobs_bwd = None
             fwd = _time(lambda: ag.spmm(vals.detach(), Xg.detach(), pattern), reps=10)
             # accuracy of the gradients against float64
             fwd_bwd()
@@ -201,6 +207,7 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "shar
             pool_ms = _time(pool_fb, reps=10)
             _emit(out, {"leg": f"gcn_131072_N128_{str(dtype).split('.')[-1]}", "nnz": mg.nnz,
                         "fwd_ms": round(fwd, 4), "fwd_bwd_ms": round(ms, 4), "fwd_bwd_separate_ms": round(ms_sep, 4),
+                        "fwd_bwd_tuned_bwd_ms": round(ms_tuned, 4), "bwd_knobs_tuned": tb[0]["knobs"],
                         "transpose_ms_once": round(t_pattern, 3),
                         "meanpool_fwd_bwd_ms": round(pool_ms, 4),
                         "err_dX": oracle.normalized_error(Xg.grad.float().cpu().numpy(), dX64, dXabs),

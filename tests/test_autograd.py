@@ -132,3 +132,46 @@ def test_gpu_segment_pool_gradients(gpu, mean):
     out.backward(torch.from_numpy(d).to(gpu))
     expect = d[idx] / (counts.numpy()[idx][:, None] if mean else 1.0)
     np.testing.assert_allclose(src.grad.cpu().numpy(), expect, rtol=1e-6, atol=1e-7)
+
+
+def test_backward_candidates_cover_n():
+    """Every candidate shape of the fused-backward tuner covers N with at most 64 lanes,
+    and the built-in shape comes first."""
+    from segscan_amd import autotune
+    for N in (1, 7, 64, 128, 200, 512):
+        cands = autotune.backward_candidates(N)
+        assert cands[0] is None
+        for k in cands[1:]:
+            assert -(-N // k["coarsen_factor"]) <= 64 and k["thread_nz"] in (8, 16)
+    assert len(autotune.backward_candidates(128)) == 7  # 2, 4 and 8 columns per lane (64, 32, 16 lanes)
+    assert len(autotune.backward_candidates(512)) == 3  # only 8 columns per lane fit in 64 lanes
+
+
+@pytest.mark.gpu
+def test_gpu_backward_tuner_and_tuned_shape(gpu, oracle, subgraph):
+    """tune_backward times every candidate shape (fastest first); the autograd layer
+    with the tuned shape gives the same float64-checked gradients."""
+    from segscan_amd import autograd as ag, autotune
+    m = subgraph
+    rng = np.random.default_rng(4)
+    A = m.to_device(gpu)
+    N = 128
+    B32 = rng.standard_normal((m.ncol, N)).astype(np.float32)
+    dC = rng.standard_normal((m.nrow, N)).astype(np.float32)
+    pattern = ag.SpmmPattern(A)
+    rows = autotune.tune_backward(pattern.T, A.vals, torch.from_numpy(dC).to(gpu), torch.from_numpy(B32).to(gpu),
+                                  autotune.backward_candidates(N), reps=2)
+    ms = [r["ms"] for r in rows]
+    assert ms == sorted(ms) and np.isfinite(ms[0]) and len(rows) == len(autotune.backward_candidates(N))
+    for knobs in (rows[0]["knobs"], {"coarsen_factor": 8, "thread_nz": 8}):
+        pattern = ag.SpmmPattern(A, knobs_bwd=knobs)
+        B = torch.from_numpy(B32).to(gpu).requires_grad_()
+        vals = torch.from_numpy(m.vals).to(gpu).requires_grad_()
+        ag.spmm(vals, B, pattern).backward(torch.from_numpy(dC).to(gpu))
+        colptr, rowind, perm = oracle.transpose(m.nrow, m.ncol, m.rowptr, m.colind)
+        dB64 = oracle.spmm_f64d(colptr, rowind, m.vals.astype(np.float64)[perm], dC.astype(np.float64))
+        dBabs = oracle.spmm_f64d(colptr, rowind, np.abs(m.vals.astype(np.float64))[perm], np.abs(dC.astype(np.float64)))
+        assert oracle.normalized_error(B.grad.cpu().numpy(), dB64, dBabs) <= 1e-6
+        dv64 = oracle.sddmm_f64d(m.rowptr, m.colind, dC.astype(np.float64), B32.astype(np.float64))
+        dvabs = oracle.sddmm_f64d(m.rowptr, m.colind, np.abs(dC.astype(np.float64)), np.abs(B32.astype(np.float64)))
+        assert oracle.normalized_error(vals.grad.cpu().numpy(), dv64, dvabs) <= 1e-6
