@@ -45,7 +45,8 @@ class BatchedGcnStep:
     dvals = SDDMM(dH_total, X) on A's pattern."""
 
     def __init__(self, batch: GraphBatch, N: int, dtype=torch.float32, variant="auto", knobs=None,
-                 variant_t="auto", knobs_t=None, seg_variant="auto", seg_knobs=None, fused: bool = True):
+                 variant_t="auto", knobs_t=None, seg_variant="auto", seg_knobs=None, fused: bool = True,
+                 knobs_bwd=None):
         self.b = batch
         A = batch.A
         dev = A.rowptr.device
@@ -66,9 +67,10 @@ class BatchedGcnStep:
         self.ws_t = ops.spmm_workspace(self.At, N, torch.float32, variant_t, knobs_t)
         # fused backward (dX and dvals from one walk over the transpose) where N fits one group of lanes
         self.fused, self.ws_b = False, None
+        self.knobs_bwd = knobs_bwd  # fused backward shape (e.g. from autotune.tune_backward); None = built-in
         if fused:
             try:
-                self.ws_b = ops.spmm_backward_workspace(T, N, torch.float32)
+                self.ws_b = ops.spmm_backward_workspace(T, N, torch.float32, knobs_bwd)
                 self.fused = True
             except _abi.SegscanError as exc:
                 if exc.status != _abi.EUNSUPPORTED:
@@ -89,7 +91,7 @@ class BatchedGcnStep:
             # the batch's values are fixed: read them in the transpose's order (cached at set-up);
             # a 16-bit X is read as stored
             ops.spmm_backward(self.T, None, self.dHt, X, dB=self.dX, dvals=self.dvals, workspace=self.ws_b,
-                              vals_t=self.At.vals)
+                              vals_t=self.At.vals, knobs=self.knobs_bwd)
             return
         Xf = X if X.dtype == torch.float32 else X.float()
         ops.spmm(self.At, self.dHt, self.variant_t, self.knobs_t, out=self.dX, workspace=self.ws_t)

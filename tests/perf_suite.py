@@ -251,6 +251,12 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "shar
         eager_ms = _time(lambda: step.run(X, dP), reps=10)
         g = step.capture(X, dP)
         graph_ms = _time(g.replay, reps=20)
+        # the fused backward's shape tuned for this batch (the built-in one is a candidate)
+        step.run(X, dP)
+        tb = autotune.tune_backward(step.T, None, step.dHt, X, autotune.backward_candidates(N), reps=5,
+                                    vals_t=step.At.vals)
+        step_t = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, variant=v, knobs=k, knobs_bwd=tb[0]["knobs"])
+        graph_tuned_ms = _time(step_t.capture(X, dP).replay, reps=20)
         sep = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, variant=v, knobs=k, fused=False)
         g_sep = sep.capture(X, dP)
         graph_sep_ms = _time(g_sep.replay, reps=20)  # transposed SpMM + SDDMM as two kernels
@@ -268,6 +274,8 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "shar
         _emit(out, {"leg": "batched_256_graphs_N64_bf16", "nodes": csr.nrow, "nnz": csr.nnz,
                     "tuned": {"variant": v, "knobs": k}, "fwd_bwd_eager_ms": round(eager_ms, 4),
                     "fwd_bwd_hipgraph_ms": round(graph_ms, 4), "fwd_bwd_hipgraph_separate_ms": round(graph_sep_ms, 4),
+                    "fwd_bwd_hipgraph_tuned_bwd_ms": round(graph_tuned_ms, 4), "bwd_knobs_tuned": tb[0]["knobs"],
+                    "bwd_tuning": [{"knobs": r["knobs"], "ms": round(r["ms"], 4)} for r in tb],
                     "fwd_pool_batched_ms": round(fwd_ms, 4),
                     "fwd_pool_per_graph_loop_plain_ms": round(loop_ms, 4),
                     "fwd_pool_per_graph_loop_auto_ms": round(loop_auto_ms, 4),
