@@ -165,7 +165,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this single-GPU build "
-              "(the row-block sharding design is described in DESIGN.md)", file=sys.stderr)
+              "(across GPUs, A would be split into nnz-balanced row blocks with all of B on every GPU; "
+              "tests/perf_suite.py's shard leg times one GPU's block)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
