@@ -241,6 +241,20 @@ segscan_status_t segscan_spmm_backward_csr(const segscan_csr_t* At, const int32_
                                            float* dB, int64_t ld_db, float* dvals,
                                            const segscan_knobs_t* knobs, uint32_t flags,
                                            void* workspace, size_t workspace_bytes, void* stream);
+/* segscan_spmm_backward_csr with dC given through a row map: A's row r reads
+ * dC[dc_map[r], :] (dc_map has At.ncol = M entries, each a valid row of dC, which
+ * may have any number of rows; the map is not checked on the device). The
+ * backward of C = A.B followed by a per-segment sum pooling P = segment_sum(C,
+ * index) takes dC = dP[index] this way: dP (one row per segment) is read in place
+ * of an E x N gathered copy, so the gather kernel and its HBM traffic go away.
+ * Everything else is as segscan_spmm_backward_csr (same workspace size). No
+ * counterpart in the reference. */
+segscan_status_t segscan_spmm_backward_csr_rowmap(const segscan_csr_t* At, const int32_t* perm, const float* vals,
+                                                  int64_t N, const void* dC, int64_t ld_dc, segscan_dtype_t dc_dtype,
+                                                  const int32_t* dc_map, const void* B, int64_t ld_b,
+                                                  segscan_dtype_t b_dtype, float* dB, int64_t ld_db, float* dvals,
+                                                  const segscan_knobs_t* knobs, uint32_t flags, void* workspace,
+                                                  size_t workspace_bytes, void* stream);
 /* Gather (backward of segment_coo): dsrc[e, :] = ddst[index[e], :] * scale(index[e]),
  * scale = 1 / max(counts[s], 1) when counts != NULL (mean pooling), else 1. */
 segscan_status_t segscan_segment_gather(const float* ddst, int64_t ld_ddst, const int32_t* index, int64_t E,

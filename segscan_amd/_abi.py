@@ -119,6 +119,9 @@ _SIGNATURES = {
     "segscan_spmm_backward_csr": (c_int32, [POINTER(CsrDesc), c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int32,
                                             c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p, POINTER(Knobs),
                                             c_uint32, c_void_p, c_size_t, c_void_p]),
+    "segscan_spmm_backward_csr_rowmap": (c_int32, [POINTER(CsrDesc), c_void_p, c_void_p, c_int64, c_void_p, c_int64,
+                                                   c_int32, c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_int64,
+                                                   c_void_p, POINTER(Knobs), c_uint32, c_void_p, c_size_t, c_void_p]),
     "segscan_segment_gather": (c_int32, [c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_int64, c_void_p]),
     "segscan_segment_counts": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
     "segscan_check_csr": (c_int32, [POINTER(CsrDesc), c_void_p, c_void_p]),

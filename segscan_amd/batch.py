@@ -84,6 +84,12 @@ class BatchedGcnStep:
         ops.spmm(A, X, self.variant, self.knobs, out=self.H, workspace=self.ws_f)
         ops.segment_coo(self.H, self.b.graph_index, self.b.n_graphs, self.seg_variant, self.seg_knobs, out=self.P,
                         workspace=self.ws_s)
+        if self.fused and dH is None:
+            # dH_total = dP[graph]: the fused backward reads dP through the graph index, so the
+            # gathered copy (nodes x N) is never written
+            ops.spmm_backward(self.T, None, dP, X, dB=self.dX, dvals=self.dvals, workspace=self.ws_b,
+                              vals_t=self.At.vals, knobs=self.knobs_bwd, dc_map=self.b.graph_index)
+            return
         ops.segment_gather(dP, self.b.graph_index, out=self.dHt)
         if dH is not None:
             self.dHt.add_(dH)

@@ -117,6 +117,7 @@ __device__ __forceinline__ void bwd_walk(const BwdParams& p, int32_t* bwd_smem) 
       const bool vn = off_n + g_lane < clen;
       const int64_t en = c0 + off_n + g_lane;
       pk = vn ? p.colind[en] : 0;
+      if (p.dc_map != nullptr && vn) pk = p.dc_map[pk];  // dC row of A's row pk
       ppos = vn ? p.perm[en] : 0;
       if (p.vals_t) pvt = vn ? p.vals_t[en] : 1.0f;
       if (off_n < clen) {

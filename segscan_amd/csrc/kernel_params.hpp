@@ -92,8 +92,9 @@ struct BwdParams {
   const int32_t* perm;     // At position -> A position
   const float* vals_t;     // values in At order, nullable (then vals through perm)
   const float* vals;       // A's values in A's order, nullable (all ones when vals_t is null too)
-  const void* dC;          // gathered rows, M x N
+  const void* dC;          // gathered rows, M x N (or indexed through dc_map)
   int64_t ld_dc;
+  const int32_t* dc_map;   // nullable: A's row r reads dC row dc_map[r]
   const void* B;           // row j of At reads B[j], K x N
   int64_t ld_b;
   float* dB;               // K x N, nullable (not computed)

@@ -1088,11 +1088,12 @@ segscan_status_t segscan_spmm_backward_workspace_size(const segscan_csr_t* At, i
   return ok();
 }
 
-segscan_status_t segscan_spmm_backward_csr(const segscan_csr_t* At, const int32_t* perm, const float* vals, int64_t N,
-                                           const void* dC, int64_t ld_dc, segscan_dtype_t dc_dtype, const void* B,
-                                           int64_t ld_b, segscan_dtype_t b_dtype, float* dB, int64_t ld_db,
-                                           float* dvals, const segscan_knobs_t* knobs, uint32_t flags,
-                                           void* workspace, size_t workspace_bytes, void* stream) {
+// Both fused-backward entry points: dc_map NULL (dC row = A's row) or a row map.
+static segscan_status_t spmm_backward_impl(const segscan_csr_t* At, const int32_t* perm, const float* vals, int64_t N,
+                                    const void* dC, int64_t ld_dc, segscan_dtype_t dc_dtype, const int32_t* dc_map,
+                                    const void* B, int64_t ld_b, segscan_dtype_t b_dtype, float* dB, int64_t ld_db,
+                                    float* dvals, const segscan_knobs_t* knobs, uint32_t flags, void* workspace,
+                                    size_t workspace_bytes, void* stream) {
   segscan_status_t st = validate_csr(At, true);
   if (st) return st;
   if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N = %lld out of range", (long long)N);
@@ -1146,6 +1147,7 @@ segscan_status_t segscan_spmm_backward_csr(const segscan_csr_t* At, const int32_
   p.rowptr = At->rowptr;
   p.colind = At->colind;
   p.perm = perm;
+  p.dc_map = dc_map;
   p.vals_t = At->vals;
   p.vals = At->vals ? nullptr : vals;
   p.dC = dC;
@@ -1182,6 +1184,26 @@ segscan_status_t segscan_spmm_backward_csr(const segscan_csr_t* At, const int32_
     return fail(SEGSCAN_EUNSUPPORTED, "no compiled backward kernel for coarsen %d, group %d", pl.CF, pl.G);
   if (e == hipSuccess && dB) e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, dB, ld_db, s);
   return hip_status(e, "fused backward");
+}
+
+segscan_status_t segscan_spmm_backward_csr(const segscan_csr_t* At, const int32_t* perm, const float* vals, int64_t N,
+                                           const void* dC, int64_t ld_dc, segscan_dtype_t dc_dtype, const void* B,
+                                           int64_t ld_b, segscan_dtype_t b_dtype, float* dB, int64_t ld_db,
+                                           float* dvals, const segscan_knobs_t* knobs, uint32_t flags,
+                                           void* workspace, size_t workspace_bytes, void* stream) {
+  return spmm_backward_impl(At, perm, vals, N, dC, ld_dc, dc_dtype, nullptr, B, ld_b, b_dtype, dB, ld_db, dvals, knobs,
+                            flags, workspace, workspace_bytes, stream);
+}
+
+segscan_status_t segscan_spmm_backward_csr_rowmap(const segscan_csr_t* At, const int32_t* perm, const float* vals,
+                                                  int64_t N, const void* dC, int64_t ld_dc, segscan_dtype_t dc_dtype,
+                                                  const int32_t* dc_map, const void* B, int64_t ld_b,
+                                                  segscan_dtype_t b_dtype, float* dB, int64_t ld_db, float* dvals,
+                                                  const segscan_knobs_t* knobs, uint32_t flags, void* workspace,
+                                                  size_t workspace_bytes, void* stream) {
+  if (!dc_map && At && At->nnz > 0 && N > 0) return fail(SEGSCAN_EINVAL, "dc_map is NULL");
+  return spmm_backward_impl(At, perm, vals, N, dC, ld_dc, dc_dtype, dc_map, B, ld_b, b_dtype, dB, ld_db, dvals, knobs,
+                            flags, workspace, workspace_bytes, stream);
 }
 
 segscan_status_t segscan_segment_gather(const float* ddst, int64_t ld_ddst, const int32_t* index, int64_t E, int64_t N,

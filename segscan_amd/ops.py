@@ -230,19 +230,28 @@ def spmm_backward_workspace(T: CscMatrix, N: int, dtype=torch.float32, knobs=Non
 def spmm_backward(T: CscMatrix, vals: Optional[torch.Tensor], dC: torch.Tensor, B: torch.Tensor, want_dB: bool = True,
                   want_dvals: bool = True, dB: Optional[torch.Tensor] = None, dvals: Optional[torch.Tensor] = None,
                   knobs=None, accumulate: bool = False, workspace: Optional[torch.Tensor] = None, stream=None,
-                  vals_t: Optional[torch.Tensor] = None):
+                  vals_t: Optional[torch.Tensor] = None, dc_map: Optional[torch.Tensor] = None):
     """Fused backward of C = A @ B over T = csr_transpose(A), one gather of dC per entry:
     dB = A^T @ dC (K x N, fp32) and dvals[p] = <dC[row(p)], B[colind[p]]> (nnz, fp32, A's order).
     vals: A's values in A's order (None = ones), read through T.perm; vals_t: the same values
     in the transpose's order (e.g. T.t.vals while A's values are unchanged), read coalesced
-    and used instead of vals. Returns (dB or None, dvals or None)."""
+    and used instead of vals. dc_map: int32, one entry per row of A; row r of A then reads
+    dC[dc_map[r]] (dC may have any number of rows), e.g. dC = dP with dc_map = the pooling
+    index, for the backward of a per-segment sum pooling without the gathered copy.
+    Returns (dB or None, dvals or None)."""
     _require_device(dC, B)
     lib = _abi.load()
     if dC.dtype != B.dtype and dC.dtype != torch.float32:
         raise TypeError("dC and B must share a dtype, or dC be float32")
     K, M, nnz = T.t.nrow, T.t.ncol, T.t.nnz
     N = B.size(1)
-    if tuple(B.shape) != (K, N) or tuple(dC.shape) != (M, N):
+    if dc_map is not None:
+        _require_device(dc_map)
+        if dc_map.dtype != torch.int32 or dc_map.numel() != M or not dc_map.is_contiguous():
+            raise ValueError(f"dc_map must be contiguous int32 with {M} entries")
+        if tuple(B.shape) != (K, N) or dC.dim() != 2 or dC.size(1) != N:
+            raise ValueError(f"B must be ({K}, N) and dC (rows, N)")
+    elif tuple(B.shape) != (K, N) or tuple(dC.shape) != (M, N):
         raise ValueError(f"B must be ({K}, N) and dC ({M}, N)")
     if want_dB and dB is None:
         dB = torch.empty((K, N), dtype=torch.float32, device=B.device)
@@ -259,11 +268,15 @@ def spmm_backward(T: CscMatrix, vals: Optional[torch.Tensor], dC: torch.Tensor, 
     if workspace is None:
         workspace = spmm_backward_workspace(T, N, dC.dtype, knobs, B.device) if dB is not None else None
     desc = T.t.with_vals(vals_t).desc()  # the At-order values, if any, travel in At.vals
-    st = lib.segscan_spmm_backward_csr(byref(desc), _ptr(T.perm), _ptr(vals), N, _ptr(dC), _leading_dim(dC, "dC"),
-                                       _dtype_code(dC), _ptr(B), _leading_dim(B, "B"), _dtype_code(B), _ptr(dB),
-                                       _leading_dim(dB, "dB") if dB is not None else 0, _ptr(dvals), kp,
-                                       _abi.ACCUMULATE if accumulate else 0, _ptr(workspace),
-                                       0 if workspace is None else workspace.numel(), _stream(stream))
+    tail = (_ptr(B), _leading_dim(B, "B"), _dtype_code(B), _ptr(dB), _leading_dim(dB, "dB") if dB is not None else 0,
+            _ptr(dvals), kp, _abi.ACCUMULATE if accumulate else 0, _ptr(workspace),
+            0 if workspace is None else workspace.numel(), _stream(stream))
+    if dc_map is not None:
+        st = lib.segscan_spmm_backward_csr_rowmap(byref(desc), _ptr(T.perm), _ptr(vals), N, _ptr(dC),
+                                                  _leading_dim(dC, "dC"), _dtype_code(dC), _ptr(dc_map), *tail)
+    else:
+        st = lib.segscan_spmm_backward_csr(byref(desc), _ptr(T.perm), _ptr(vals), N, _ptr(dC), _leading_dim(dC, "dC"),
+                                           _dtype_code(dC), *tail)
     _abi.check(st, "spmm_backward")
     return dB, dvals
 

@@ -158,6 +158,27 @@ def test_fused_backward_validation_and_plan(lib):
     assert n.value == (4194304 // 32) * 4 + (4194304 // 32) * 64 * 4
 
 
+def test_fused_backward_rowmap_validation(lib):
+    """The row-map entry point needs the map and otherwise validates as the plain one."""
+    d = _desc(nrow=10, ncol=12, nnz=20)
+    f = lib.segscan_spmm_backward_csr_rowmap
+
+    def call(perm=0x10, N=8, dC=0x100, dc_map=0x180, dc_t=_abi.F32, B=0x200, dB=0x300, ws=0x500):
+        return f(byref(d), perm, None, N, dC, 8, dc_t, dc_map, B, 8, _abi.F32, dB, 8, 0x400, None, 0, ws, 1 << 30,
+                 None)
+
+    assert call(dc_map=None) == _abi.EINVAL
+    assert "dc_map" in lib.segscan_last_error().decode()
+    assert call(perm=None) == _abi.EINVAL
+    assert call(dC=None) == _abi.EINVAL
+    assert call(dc_t=_abi.BF16) == _abi.EUNSUPPORTED  # bf16 dC with an fp32 B
+    assert call(ws=None) == _abi.EWORKSPACE
+    # nothing to gather (N = 0): no map needed, dvals zero-filled on the device
+    e = _desc(nrow=10, ncol=12, nnz=0)
+    assert f(byref(e), None, None, 8, None, 8, _abi.F32, None, None, 8, _abi.F32, None, 8, None, None, 0, None, 0,
+             None) == _abi.OK
+
+
 def _describe(lib, nrow, nnz, N, variant, dtype=_abi.F32, max_row=0, **knobs):
     info = _abi.LaunchInfo()
     d = _desc(nrow=nrow, ncol=nrow, nnz=nnz, max_row=max_row)

@@ -226,3 +226,36 @@ def test_fused_backward_short_rows_bf16_operand(gpu, dc_dtype, N, knobs):
     want_dv = np.einsum("pn,pn->p", dC[rows].astype(np.float64), B[m.colind].astype(np.float64))
     assert np.array_equal(dB.cpu().numpy(), want_dB.astype(np.float32))
     assert np.array_equal(dv.cpu().numpy(), want_dv.astype(np.float32))
+
+
+@pytest.mark.parametrize("N,b_dtype,knobs", [
+    (64, torch.float32, None),
+    (64, torch.bfloat16, None),                # short rows + bf16 B: the row-batch form
+    (100, torch.float32, {"thread_nz": 1}),    # carries at nearly every chunk, a partial lane
+    (32, torch.bfloat16, {"block_numer": 1, "block_denom": 1000000}),
+])
+def test_fused_backward_rowmap_equals_gathered(gpu, N, b_dtype, knobs):
+    """dC through a row map (A's row r reads dP[map[r]]) gives what the plain entry point
+    gives on the gathered copy dP[map]: dvals bit for bit (same products in the same
+    order), dB up to the order of its atomic carries; both exact on integer operands."""
+    m = _short_row_matrix(N + 3)
+    rng = np.random.default_rng(N + 1)
+    S = 37
+    gmap = np.sort(rng.integers(0, S, size=m.nrow)).astype(np.int32)  # e.g. a pooling index
+    dP = rng.integers(-3, 4, size=(S, N)).astype(np.float32)
+    B = rng.integers(-3, 4, size=(m.ncol, N)).astype(np.float32)
+    A = m.to_device(gpu)
+    T = ops.csr_transpose(A)
+    Bg = torch.from_numpy(B).to(gpu).to(b_dtype)
+    dPg = torch.from_numpy(dP).to(gpu)
+    mapg = torch.from_numpy(gmap).to(gpu)
+    dB1, dv1 = ops.spmm_backward(T, A.vals, dPg, Bg, knobs=knobs, dc_map=mapg)
+    dB2, dv2 = ops.spmm_backward(T, A.vals, dPg[mapg.long()].contiguous(), Bg, knobs=knobs)
+    assert torch.equal(dv1, dv2)
+    assert torch.equal(dB1, dB2)  # integer operands: every sum is exact
+    rows = m.row_ids()
+    want = np.zeros((m.ncol, N))
+    np.add.at(want, m.colind, m.vals[:, None].astype(np.float64) * dP[gmap[rows]])
+    assert np.array_equal(dB1.cpu().numpy(), want.astype(np.float32))
+    with pytest.raises(ValueError):
+        ops.spmm_backward(T, A.vals, dPg, Bg, dc_map=mapg[:-1])
