@@ -60,7 +60,9 @@ class BatchedGcnStep:
         self.At = T.t.measured().with_vals(None if A.vals is None else A.vals.index_select(0, T.perm.long()))
         self.H = torch.empty((A.nrow, N), dtype=torch.float32, device=dev)
         self.P = torch.empty((batch.n_graphs, N), dtype=torch.float32, device=dev)
-        self.dHt = torch.empty((A.nrow, N), dtype=torch.float32, device=dev)
+        # dP gathered per node (+ dH): needed only with a dH of its own or the separate kernels
+        # (nodes x N fp32), so it is allocated on first use (the eager run before a capture)
+        self.dHt = None
         self.dX = torch.empty((A.ncol, N), dtype=torch.float32, device=dev)
         self.dvals = torch.empty(max(A.nnz, 1), dtype=torch.float32, device=dev)[: A.nnz]
         self.ws_f = ops.spmm_workspace(A, N, dtype, variant, knobs)
@@ -90,6 +92,8 @@ class BatchedGcnStep:
             ops.spmm_backward(self.T, None, dP, X, dB=self.dX, dvals=self.dvals, workspace=self.ws_b,
                               vals_t=self.At.vals, knobs=self.knobs_bwd, dc_map=self.b.graph_index)
             return
+        if self.dHt is None:
+            self.dHt = torch.empty((A.nrow, self.N), dtype=torch.float32, device=dP.device)
         ops.segment_gather(dP, self.b.graph_index, out=self.dHt)
         if dH is not None:
             self.dHt.add_(dH)

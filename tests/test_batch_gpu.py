@@ -95,3 +95,31 @@ def test_batched_bf16_features(gpu, oracle, graphs):
     Xq = X.float().cpu().numpy()
     H64, D = oracle.spmm_f64(csr.rowptr, csr.colind, csr.vals, Xq)
     assert oracle.normalized_error(step.H.cpu().numpy(), H64, D) <= 1e-6
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_batched_step_with_extra_dH(gpu, oracle, graphs, fused):
+    """With a node gradient dH of its own the step forms dP[graph] + dH explicitly (the
+    row-map backward does not apply); fused=False runs the transposed SpMM and SDDMM
+    as separate kernels. Both against the oracle, eager and replayed."""
+    csr, offsets = graphs
+    b = batch.make_batch(csr, offsets, gpu)
+    rng = np.random.default_rng(7)
+    N = 32
+    step = batch.BatchedGcnStep(b, N, fused=fused)
+    Xh = rng.uniform(-1, 1, size=(csr.nrow, N)).astype(np.float32)
+    dPh = rng.standard_normal((b.n_graphs, N)).astype(np.float32)
+    dHh = rng.standard_normal((csr.nrow, N)).astype(np.float32)
+    X, dP, dH = (torch.from_numpy(a).to(gpu) for a in (Xh, dPh, dHh))
+    gi = np.repeat(np.arange(offsets.size - 1), np.diff(offsets))
+    dtot = dPh[gi] + dHh
+    colptr, rowind, perm = oracle.transpose(csr.nrow, csr.ncol, csr.rowptr, csr.colind)
+    dX64, dXabs = oracle.spmm_f64(colptr, rowind, csr.vals[perm], dtot)
+    dv64, dvabs = oracle.sddmm_f64(csr.rowptr, csr.colind, dtot, Xh)
+    g = step.capture(X, dP, dH)
+    for run in (lambda: step.run(X, dP, dH), g.replay):
+        step.dX.fill_(-7.0)
+        run()
+        torch.cuda.synchronize()
+        assert oracle.normalized_error(step.dX.cpu().numpy(), dX64, dXabs) <= 1e-6
+        assert oracle.normalized_error(step.dvals.cpu().numpy(), dv64, dvabs) <= 1e-6
