@@ -50,6 +50,24 @@ def counted_traffic(variant_id: int, knobs: dict | None, N: int) -> dict | None:
     return None
 
 
+def host_cpu() -> dict:
+    """The host's CPU model (/proc/cpuinfo, as lscpu reports it) and the cores this
+    process may run on."""
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "nproc": usable, "logical_cpus": os.cpu_count()}
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -358,10 +376,10 @@ def main():
             times.append(time.perf_counter() - t1)
         cpu_s = float(np.median(times))
         cpu = {"value": round(b_alg(4) / cpu_s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-               "ms": round(cpu_s * 1e3, 1),
+               "ms": round(cpu_s * 1e3, 1), "host": host_cpu(),
                "sample": "full workload (262144x262144, %d nnz, N=%d), fp32 i->p->j loop, 1 warm-up + median of 25"
                          % (m.nnz, N)}
-        log(f"cpu baseline: {cpu_s * 1e3:.1f} ms ({cpu['value']} GB/s), host {os.cpu_count()} cpus")
+        log(f"cpu baseline: {cpu_s * 1e3:.1f} ms ({cpu['value']} GB/s), host {cpu['host']}")
 
     achieved = b_alg(4) / (kern_ms * 1e-3) / 1e9
     counted = counted_traffic(info32["variant"], k32, N)

@@ -270,9 +270,12 @@ def backward_candidates(N: int) -> list[dict | None]:
 
 
 def tune_backward(T: ops.CscMatrix, vals: torch.Tensor | None, dC: torch.Tensor, B: torch.Tensor,
-                  candidates: Iterable, reps: int = 10, vals_t: torch.Tensor | None = None) -> list[dict]:
+                  candidates: Iterable, reps: int = 10, vals_t: torch.Tensor | None = None,
+                  dc_map: torch.Tensor | None = None) -> list[dict]:
     """Times the fused backward (both dB and dvals) over knob points, fastest first;
-    points the library rejects are kept with ms = inf and the error."""
+    points the library rejects are kept with ms = inf and the error. With dc_map, dC
+    is read through the row map exactly as the caller will run it (e.g. dC = dP with
+    the pooling index), so the shape is tuned for that access pattern."""
     N = B.size(1)
     dB = torch.empty((T.t.nrow, N), dtype=torch.float32, device=B.device)
     dv = torch.empty(max(T.t.nnz, 1), dtype=torch.float32, device=B.device)[: T.t.nnz]
@@ -281,7 +284,7 @@ def tune_backward(T: ops.CscMatrix, vals: torch.Tensor | None, dC: torch.Tensor,
         try:
             ws = ops.spmm_backward_workspace(T, N, dC.dtype, knobs, B.device)
             ms = time_call(lambda: ops.spmm_backward(T, vals, dC, B, dB=dB, dvals=dv, knobs=knobs, workspace=ws,
-                                                     vals_t=vals_t), reps)
+                                                     vals_t=vals_t, dc_map=dc_map), reps)
         except _abi.SegscanError as exc:  # unsupported knob points are skipped, not fatal
             results.append({"knobs": knobs, "ms": float("inf"), "error": str(exc)})
             continue

@@ -83,6 +83,12 @@ class BatchedGcnStep:
 
     def run(self, X: torch.Tensor, dP: torch.Tensor, dH: torch.Tensor | None = None) -> None:
         A = self.b.A
+        if tuple(X.shape) != (A.ncol, self.N) or X.dtype != self.dtype:
+            raise ValueError(f"X must be {self.dtype} of shape ({A.ncol}, {self.N})")
+        if tuple(dP.shape) != (self.b.n_graphs, self.N) or dP.dtype != torch.float32:
+            raise ValueError(f"dP must be float32 of shape ({self.b.n_graphs}, {self.N})")
+        if dH is not None and (tuple(dH.shape) != (A.nrow, self.N) or dH.dtype != torch.float32):
+            raise ValueError(f"dH must be float32 of shape ({A.nrow}, {self.N})")
         ops.spmm(A, X, self.variant, self.knobs, out=self.H, workspace=self.ws_f)
         ops.segment_coo(self.H, self.b.graph_index, self.b.n_graphs, self.seg_variant, self.seg_knobs, out=self.P,
                         workspace=self.ws_s)

@@ -249,8 +249,8 @@ def spmm_backward(T: CscMatrix, vals: Optional[torch.Tensor], dC: torch.Tensor, 
         _require_device(dc_map)
         if dc_map.dtype != torch.int32 or dc_map.numel() != M or not dc_map.is_contiguous():
             raise ValueError(f"dc_map must be contiguous int32 with {M} entries")
-        if tuple(B.shape) != (K, N) or dC.dim() != 2 or dC.size(1) != N:
-            raise ValueError(f"B must be ({K}, N) and dC (rows, N)")
+        if tuple(B.shape) != (K, N) or dC.dim() != 2 or dC.size(1) != N or (M > 0 and dC.size(0) < 1):
+            raise ValueError(f"B must be ({K}, N) and dC (rows >= 1, N)")
     elif tuple(B.shape) != (K, N) or tuple(dC.shape) != (M, N):
         raise ValueError(f"B must be ({K}, N) and dC ({M}, N)")
     if want_dB and dB is None:

@@ -251,11 +251,10 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "shar
         eager_ms = _time(lambda: step.run(X, dP), reps=10)
         g = step.capture(X, dP)
         graph_ms = _time(g.replay, reps=20)
-        # the fused backward's shape tuned for this batch (the built-in one is a candidate)
-        dHt = ops.segment_gather(dP, b.graph_index)  # the gathered gradient, as the separate path forms it
-        tb = autotune.tune_backward(step.T, None, dHt, X, autotune.backward_candidates(N), reps=5,
-                                    vals_t=step.At.vals)
-        del dHt
+        # the fused backward's shape tuned for this batch (the built-in one is a candidate), on
+        # dP read through the graph index exactly as the step runs it
+        tb = autotune.tune_backward(step.T, None, dP, X, autotune.backward_candidates(N), reps=5,
+                                    vals_t=step.At.vals, dc_map=b.graph_index)
         step_t = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, variant=v, knobs=k, knobs_bwd=tb[0]["knobs"])
         graph_tuned_ms = _time(step_t.capture(X, dP).replay, reps=20)
         sep = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, variant=v, knobs=k, fused=False)

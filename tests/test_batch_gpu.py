@@ -84,17 +84,41 @@ def test_per_graph_loop_matches(gpu, oracle, graphs):
 
 
 def test_batched_bf16_features(gpu, oracle, graphs):
+    """bf16 X (the configuration of the 256-graph measurement): H, dX and dvals against
+    the float64 oracle on the rounded X, eager and after capture + replays."""
     csr, offsets = graphs
     b = batch.make_batch(csr, offsets, gpu)
     rng = np.random.default_rng(2)
     N = 64
     X = torch.from_numpy(rng.uniform(-1, 1, size=(csr.nrow, N)).astype(np.float32)).to(torch.bfloat16).to(gpu)
-    dP = torch.from_numpy(rng.standard_normal((b.n_graphs, N)).astype(np.float32)).to(gpu)
+    dPh = rng.standard_normal((b.n_graphs, N)).astype(np.float32)
+    dP = torch.from_numpy(dPh).to(gpu)
     step = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16)
-    step.run(X, dP)
     Xq = X.float().cpu().numpy()
     H64, D = oracle.spmm_f64(csr.rowptr, csr.colind, csr.vals, Xq)
-    assert oracle.normalized_error(step.H.cpu().numpy(), H64, D) <= 1e-6
+    gi = np.repeat(np.arange(offsets.size - 1), np.diff(offsets))
+    dHt = dPh[gi]
+    colptr, rowind, perm = oracle.transpose(csr.nrow, csr.ncol, csr.rowptr, csr.colind)
+    dX64, dXabs = oracle.spmm_f64(colptr, rowind, csr.vals[perm], dHt)
+    dv64, dvabs = oracle.sddmm_f64(csr.rowptr, csr.colind, dHt, Xq)
+
+    def check():
+        assert oracle.normalized_error(step.H.cpu().numpy(), H64, D) <= 1e-6
+        assert oracle.normalized_error(step.dX.cpu().numpy(), dX64, dXabs) <= 1e-6
+        assert oracle.normalized_error(step.dvals.cpu().numpy(), dv64, dvabs) <= 1e-6
+
+    step.run(X, dP)
+    torch.cuda.synchronize()
+    check()
+    g = step.capture(X, dP)
+    for _ in range(2):
+        step.H.fill_(9.0)
+        step.dX.fill_(-7.0)
+        step.dvals.fill_(3.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        check()
 
 
 @pytest.mark.parametrize("fused", [True, False])
