@@ -182,6 +182,23 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N,
                                   segscan_spmm_variant_t variant, const segscan_knobs_t* knobs,
                                   uint32_t flags, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Allocation-free SpMM: segscan_spmm_csr without a workspace argument. No counterpart in
+ * the reference; it is for callers that cannot allocate (code under hipGraph capture,
+ * serving loops, harnesses that forbid hipMalloc). The scratch a plan needs (ebsr
+ * carries, ebpr tile rows) comes from a static device array of
+ * SEGSCAN_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; the call allocates
+ * nothing. A plan that needs more (segscan_spmm_workspace_size) returns
+ * SEGSCAN_EWORKSPACE before any HIP call. Every call on one GPU shares that GPU's array,
+ * so calls that overlap on one GPU (concurrent streams or threads) race; calls ordered
+ * on one stream do not. `stream` must belong to the current device. Results equal
+ * segscan_spmm_csr's for the same arguments. */
+#define SEGSCAN_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+segscan_status_t segscan_spmm_csr_noalloc(const segscan_csr_t* A, int64_t N,
+                                          const void* B, int64_t ldb, segscan_dtype_t b_dtype,
+                                          float* C, int64_t ldc,
+                                          segscan_spmm_variant_t variant, const segscan_knobs_t* knobs,
+                                          uint32_t flags, void* stream);
+
 /* ---- segment_coo: dst[s, :] (=|+=) sum_{e : index[e] == s} src[e, :] ----------------
  * index: [E] int32, sorted non-decreasing, values in [0, dst_len). src: E x N, row
  * stride ld_src in [N, 2^31 - 1]. dst: dst_len x N fp32, row stride ld_dst. */

@@ -108,6 +108,8 @@ _SIGNATURES = {
     "segscan_spmm_workspace_size": (c_int32, [POINTER(CsrDesc), c_int64, c_int32, c_int32, POINTER(Knobs), POINTER(c_size_t)]),
     "segscan_spmm_csr": (c_int32, [POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p, c_int64, c_int32,
                                    POINTER(Knobs), c_uint32, c_void_p, c_size_t, c_void_p]),
+    "segscan_spmm_csr_noalloc": (c_int32, [POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p, c_int64,
+                                           c_int32, POINTER(Knobs), c_uint32, c_void_p]),
     "segscan_segment_coo_describe": (c_int32, [c_int64, c_int64, c_int64, c_int32, c_int32, POINTER(Knobs), POINTER(LaunchInfo)]),
     "segscan_segment_coo_workspace_size": (c_int32, [c_int64, c_int64, c_int64, c_int32, c_int32, POINTER(Knobs), POINTER(c_size_t)]),
     "segscan_segment_coo": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_int64, c_void_p, c_int64, c_int64,

@@ -28,6 +28,7 @@ SOURCES = [
     "rbsr_kernels.hip",
     "misc_kernels.hip",
     "segscan_abi.cpp",
+    "noalloc.hip",
     "mtx_loader.cpp",
 ]
 

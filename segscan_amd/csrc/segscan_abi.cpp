@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/segscan.h"
+#include "abi_internal.hpp"
 #include "kernel_params.hpp"
 
 namespace {
@@ -683,6 +684,21 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N, const void*
                                   segscan_dtype_t b_dtype, float* C, int64_t ldc, segscan_spmm_variant_t variant,
                                   const segscan_knobs_t* knobs, uint32_t flags, void* workspace,
                                   size_t workspace_bytes, void* stream) {
+  return segscan::spmm_csr_impl(A, N, B, ldb, b_dtype, C, ldc, variant, knobs, flags, workspace, workspace_bytes,
+                                stream, nullptr);
+}
+
+}  // extern "C"
+
+namespace segscan {
+
+// segscan_spmm_csr's body. With plan_bytes non-NULL the call stops after validation and
+// planning (no HIP call) and reports the workspace the launch would need (0 when the
+// call launches nothing or needs no workspace).
+segscan_status_t spmm_csr_impl(const segscan_csr_t* A, int64_t N, const void* B, int64_t ldb,
+                               segscan_dtype_t b_dtype, float* C, int64_t ldc, segscan_spmm_variant_t variant,
+                               const segscan_knobs_t* knobs, uint32_t flags, void* workspace, size_t workspace_bytes,
+                               void* stream, size_t* plan_bytes) {
   segscan_status_t st = validate_csr(A, true);
   if (st) return st;
   if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N = %lld out of range", (long long)N);
@@ -705,6 +721,10 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N, const void*
   SpmmPlan pl;
   st = plan_spmm(M, nnz, A->max_row_nnz, std::max<int64_t>(N, 1), b_dtype, variant, from_c(knobs), cap, pl);
   if (st) return st;
+  if (plan_bytes) {
+    *plan_bytes = (M == 0 || N == 0 || nnz == 0) ? 0 : pl.ws_bytes;
+    return ok();
+  }
   if (M == 0 || N == 0) return ok();
   if (pl.ws_bytes > 0 && nnz > 0) {
     if (!workspace) return fail(SEGSCAN_EWORKSPACE, "workspace of %zu bytes required", pl.ws_bytes);
@@ -818,6 +838,12 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N, const void*
     return fail(SEGSCAN_EUNSUPPORTED, "no compiled kernel for dtype %d, coarsen %d, group %d", (int)b_dtype, pl.CF, pl.G);
   return hip_status(e, "SpMM launch");
 }
+
+segscan_status_t set_error(segscan_status_t st, const char* msg) { return fail(st, "%s", msg); }
+
+}  // namespace segscan
+
+extern "C" {
 
 segscan_status_t segscan_segment_coo_describe(int64_t E, int64_t N, int64_t dst_len, segscan_dtype_t src_dtype,
                                               segscan_seg_variant_t variant, const segscan_knobs_t* knobs,
