@@ -83,7 +83,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-tuned", action="store_true", help="skip the comparison legs (for profiling)")
     ap.add_argument("--dump", type=str, default=None, help="write tuning results to this JSON file")
-    ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "segment_f32", "segment_bf16", "batched_graph"],
+    ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "segment_f32", "segment_bf16", "batched_graph",
+                                           "batched_graph_kernels"],
                     default=None,
                     help="run one comparison leg alone (for per-kernel profiles) and print its timing")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -109,8 +110,10 @@ def only_leg(args):
     single kernel configuration: the tuned bf16-B SpMM, the fused backward (full: dB and
     dvals, values through perm), segment_coo AUTO over the power-law index (E = 4.19M,
     N = 64), or one replay of the batched step's hipGraph (256 uneven graphs, 2.39M nodes,
-    9.58M nnz, bf16 features, N = 64: SpMM, per-graph sum pooling, pooling gather and the
-    fused backward). Prints {"leg", "ms"} (HIP events, median per call)."""
+    9.58M nnz, bf16 features, N = 64: forward SpMM, per-graph sum pooling and the backward
+    for dH = dP[graph]) as the batch plan's single launch (batched_graph) or kernel by
+    kernel (batched_graph_kernels: SpMM, segment_coo pooling, fused backward through the
+    graph index). Prints {"leg", "ms"} (HIP events, median per call)."""
     import numpy as np
     import torch
 
@@ -118,12 +121,13 @@ def only_leg(args):
     dev = torch.device("cuda:0")
     N = args.N
     rng = np.random.default_rng(0)
-    if args.only_leg == "batched_graph":
+    if args.only_leg in ("batched_graph", "batched_graph_kernels"):
         csr, offsets = synth.batched_graphs(seed=0)
         gb = batch.make_batch(csr, offsets, dev)
         X = torch.from_numpy(rng.uniform(-1, 1, size=(csr.nrow, N)).astype(np.float32)).to(dev).to(torch.bfloat16)
         dP = torch.from_numpy(rng.standard_normal((gb.n_graphs, N)).astype(np.float32)).to(dev)
-        graph = batch.BatchedGcnStep(gb, N, dtype=torch.bfloat16).capture(X, dP)
+        single = args.only_leg == "batched_graph"
+        graph = batch.BatchedGcnStep(gb, N, dtype=torch.bfloat16, single_launch=single).capture(X, dP)
         _time_leg(args, graph.replay)
         return
     m = synth.powerlaw_csr(seed=0)

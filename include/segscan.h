@@ -281,6 +281,48 @@ segscan_status_t segscan_segment_gather(const float* ddst, int64_t ld_ddst, cons
 segscan_status_t segscan_segment_counts(const int32_t* index, int64_t E, int64_t dst_len, int32_t* counts,
                                         void* stream);
 
+/* ---- batched graphs: one persistent launch per step ---------------------------------------
+ * A batch of graphs as one block-diagonal CSR A (square, M = K nodes; graph g owns rows and
+ * columns [graph_offsets[g], graph_offsets[g+1]), every entry's column in its row's graph).
+ * One run computes, for node features X (M x N of x_dtype) and the gradient dP (n_graphs x
+ * N, fp32) of the per-graph sum pooling:
+ *   H = A.X (M x N fp32)   P[g] = sum of the H rows of graph g (n_graphs x N fp32)
+ *   dX = A^T.dH and dvals = SDDMM(dH, X) on A's pattern, for dH = dP[graph]
+ * in one kernel launch (no fix-up pass, no memset; capturable in a hipGraph). With the
+ * block-diagonal structure, dX[j] = colsum(A)[j] * dP[g(j)]; the column sums are taken from
+ * A's values at plan creation (segscan_batch_plan_set_values after the values change).
+ * H, dX and dvals may be NULL (not computed); dP NULL runs the forward only. N <= 512
+ * (fp32: <= 256). Rows of X, H, P, dX and dP must be aligned to the lane's columns (as
+ * torch tensors are). The plan borrows A's device arrays (keep them alive) and owns device
+ * memory of its own (segscan_batch_plan_describe: plan_bytes), allocated at creation;
+ * creation reads rowptr back and synchronises `stream`. Runs of one plan must not overlap
+ * (they share its pool accumulator): order them on one stream. No counterpart in the
+ * reference (it has no batching); it replaces a per-graph launch loop of its SpMM and
+ * segment_coo (spmm_cg.cuh:553-591, segscan.cuh:974-1008). Knobs (NULL = built-in):
+ * coarsen_factor, group_factor (the lane shape), thread_nz (entries per group per work
+ * item / group size), block_numer / block_denom (scale of the resident grid). */
+typedef struct segscan_batch_plan* segscan_batch_plan_t;
+typedef struct segscan_batch_info {
+  int64_t nitems;          /* work items (row ranges) of the forward */
+  int64_t grid;            /* workgroups launched */
+  int32_t group_size;
+  int32_t coarsen_factor;
+  int32_t thread_per_block;
+  int64_t item_target;     /* entries + rows per item */
+  size_t plan_bytes;       /* device memory held by the plan */
+} segscan_batch_info_t;
+
+segscan_status_t segscan_batch_plan_create(const segscan_csr_t* A, const int64_t* graph_offsets, int64_t n_graphs,
+                                           int64_t N, segscan_dtype_t x_dtype, const segscan_knobs_t* knobs,
+                                           void* stream, segscan_batch_plan_t* plan);
+segscan_status_t segscan_batch_plan_set_values(segscan_batch_plan_t plan, const float* vals, void* stream);
+segscan_status_t segscan_batch_plan_describe(segscan_batch_plan_t plan, segscan_batch_info_t* info);
+segscan_status_t segscan_batch_plan_run(segscan_batch_plan_t plan, const void* X, int64_t ld_x,
+                                        const float* dP, int64_t ld_dp, float* H, int64_t ld_h,
+                                        float* P, int64_t ld_p, float* dX, int64_t ld_dx, float* dvals,
+                                        void* stream);
+segscan_status_t segscan_batch_plan_destroy(segscan_batch_plan_t plan);
+
 /* ---- device-side input checks (optional; the compute calls trust their inputs) ---------
  * Writes into *flags_dev (device int32) a bit set: 1 rowptr not monotone or not
  * spanning [0, nnz], 2 column index out of [0, ncol), 4 columns not strictly increasing in a row. */

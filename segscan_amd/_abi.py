@@ -94,6 +94,16 @@ class LaunchInfo(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("nitems", c_int64), ("grid", c_int64), ("group_size", c_int32), ("coarsen_factor", c_int32),
+        ("thread_per_block", c_int32), ("item_target", c_int64), ("plan_bytes", c_size_t),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class SegscanError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{message} (status {status})")
@@ -108,6 +118,13 @@ _SIGNATURES = {
     "segscan_spmm_workspace_size": (c_int32, [POINTER(CsrDesc), c_int64, c_int32, c_int32, POINTER(Knobs), POINTER(c_size_t)]),
     "segscan_spmm_csr": (c_int32, [POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p, c_int64, c_int32,
                                    POINTER(Knobs), c_uint32, c_void_p, c_size_t, c_void_p]),
+    "segscan_batch_plan_create": (c_int32, [POINTER(CsrDesc), POINTER(c_int64), c_int64, c_int64, c_int32,
+                                            POINTER(Knobs), c_void_p, POINTER(c_void_p)]),
+    "segscan_batch_plan_set_values": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "segscan_batch_plan_describe": (c_int32, [c_void_p, POINTER(BatchInfo)]),
+    "segscan_batch_plan_run": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
+                                         c_int64, c_void_p, c_int64, c_void_p, c_void_p]),
+    "segscan_batch_plan_destroy": (c_int32, [c_void_p]),
     "segscan_spmm_csr_noalloc": (c_int32, [POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p, c_int64,
                                            c_int32, POINTER(Knobs), c_uint32, c_void_p]),
     "segscan_segment_coo_describe": (c_int32, [c_int64, c_int64, c_int64, c_int32, c_int32, POINTER(Knobs), POINTER(LaunchInfo)]),

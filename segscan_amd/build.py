@@ -29,6 +29,7 @@ SOURCES = [
     "misc_kernels.hip",
     "segscan_abi.cpp",
     "noalloc.hip",
+    "batch_kernels.hip",
     "mtx_loader.cpp",
 ]
 

@@ -275,3 +275,35 @@ def test_workspace_size_query(lib):
     assert n.value > 0
     assert lib.segscan_csr_transpose_workspace_size(byref(d), byref(n)) == _abi.OK
     assert n.value >= 1000 * 4
+
+
+def test_batch_plan_validation(lib):
+    """segscan_batch_plan_create rejects bad arguments on the host, before any HIP call."""
+    offs = (ctypes.c_int64 * 3)(0, 4, 10)
+    plan = ctypes.c_void_p()
+
+    def create(desc, offsets=offs, n_graphs=2, N=16, dtype=_abi.F32, knobs=None):
+        kp = None if knobs is None else byref(_abi.Knobs.make(**knobs))
+        return lib.segscan_batch_plan_create(None if desc is None else byref(desc), offsets, n_graphs, N, dtype, kp,
+                                             None, byref(plan))
+
+    sq = _desc(nrow=10, ncol=10, nnz=20)
+    assert create(None) == _abi.EINVAL
+    assert create(_desc(nrow=10, ncol=12, nnz=20)) == _abi.EINVAL          # not square
+    assert create(sq, offsets=None) == _abi.EINVAL
+    assert create(sq, offsets=(ctypes.c_int64 * 3)(0, 4, 9)) == _abi.EINVAL  # does not end at M
+    assert create(sq, offsets=(ctypes.c_int64 * 3)(0, 6, 4)) == _abi.EINVAL  # decreasing (and ends wrong)
+    assert create(sq, offsets=(ctypes.c_int64 * 3)(1, 4, 10)) == _abi.EINVAL
+    assert create(sq, n_graphs=0) == _abi.EINVAL
+    assert create(sq, N=0) == _abi.EUNSUPPORTED
+    assert create(sq, N=513) == _abi.EUNSUPPORTED
+    assert create(sq, N=300) == _abi.EUNSUPPORTED                          # fp32: at most 64 lanes x 4 columns
+    assert create(sq, dtype=7) == _abi.EINVAL
+    assert create(sq, N=64, knobs=dict(group_factor=3, coarsen_factor=1)) == _abi.EUNSUPPORTED
+    assert not plan.value
+    # a NULL plan is refused by every call but destroy
+    info = _abi.BatchInfo()
+    assert lib.segscan_batch_plan_describe(None, byref(info)) == _abi.EINVAL
+    assert lib.segscan_batch_plan_run(None, 0x1000, 16, None, 0, None, 0, 0x2000, 16, None, 0, None, None) == _abi.EINVAL
+    assert lib.segscan_batch_plan_set_values(None, None, None) == _abi.EINVAL
+    assert lib.segscan_batch_plan_destroy(None) == _abi.OK

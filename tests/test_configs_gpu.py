@@ -281,9 +281,12 @@ def _batched_check(oracle, step, ref):
         assert err <= FP32_TOL, (name, err)
 
 
-def test_batched_256_graphs_bf16(gpu, oracle):
+@pytest.mark.parametrize("single_launch", [True, False])
+def test_batched_256_graphs_bf16(gpu, oracle, single_launch):
+    """The default step (one launch of the batch plan) and the kernel-by-kernel form."""
     csr, offsets = synth.batched_graphs(seed=0)
-    print(f"batched {offsets.size - 1} graphs nodes={csr.nrow} nnz={csr.nnz}", flush=True)
+    print(f"batched {offsets.size - 1} graphs nodes={csr.nrow} nnz={csr.nnz} single_launch={single_launch}",
+          flush=True)
     b = batch.make_batch(csr, offsets, gpu)
     N = 64
     rng = np.random.default_rng(40)
@@ -295,7 +298,7 @@ def test_batched_256_graphs_bf16(gpu, oracle):
         dP.copy_(torch.from_numpy(rng.standard_normal((b.n_graphs, N)).astype(np.float32)).to(gpu))
         return X.float().cpu().numpy(), dP.cpu().numpy()
 
-    step = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16)
+    step = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, single_launch=single_launch)
     Xq, dPh = new_inputs()
     step.run(X, dP)
     torch.cuda.synchronize()
