@@ -28,6 +28,7 @@
 // exchanges, zeroing it for the next run. No workgroup ever waits on another.
 #include <algorithm>
 #include <cstdio>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -135,8 +136,16 @@ __device__ __forceinline__ int butterfly_entry(int gl) {
   return e;
 }
 
+// Waves per SIMD the register allocator is held to: with up to 8 bytes of a dense row
+// per lane the kernel fits 128 VGPRs with a few spills, and 4 waves beat 3 unspilled
+// ones (256 graphs, bf16 N = 64: 0.60 against 0.64 ms); wider slices, and fp16 at 8
+// bytes (its widening takes more registers), spill too much and stay uncapped.
+template <typename VT, int CF>
+constexpr int kBatchWaves = (sizeof(VT) * CF <= 8 && !(std::is_same_v<VT, f16_t> && CF == 4)) ? 4 : 1;
+
 template <typename VT, int CF, int G>
-__global__ void __launch_bounds__(kBatchTpb) batch_step_kernel(BatchParams p) {
+__global__ void __launch_bounds__(kBatchTpb) __attribute__((amdgpu_waves_per_eu(kBatchWaves<VT, CF>)))
+batch_step_kernel(BatchParams p) {
   constexpr int NG = kBatchTpb / G;   // groups per workgroup
   constexpr int U = (G < 16) ? G : 16;  // entries per batch of dense-row loads
   constexpr int W = G * CF;           // columns one group covers (>= N)
