@@ -323,6 +323,41 @@ segscan_status_t segscan_batch_plan_run(segscan_batch_plan_t plan, const void* X
                                         void* stream);
 segscan_status_t segscan_batch_plan_destroy(segscan_batch_plan_t plan);
 
+/* ---- tuning tables and the autotuner ---------------------------------------------------
+ * Replace the reference's parameter sweeps (test-cg/src/params.md:1-438, run under Nsight by
+ * test-cg/src/test.py:42-59) and its static-table analysis (test-cg/src/static.py:15-35).
+ * A table maps keys to a variant and knobs and is stored as JSON (the format
+ * segscan_amd/autotune.py reads and writes). Keys start with segscan_tune_prefix(), the
+ * GPU architecture and the ROCm release ("gfx950|rocm7.2"); load drops entries of another
+ * architecture or release (*dropped counts them), so a table is re-tuned, not misapplied,
+ * after a ROCm upgrade. Lookup tries the matrix's own entry (rows, columns, nnz, longest
+ * row), then its bucket (N rounded up to a power of two, row-skew class, size class); with
+ * neither it returns SEGSCAN_SPMM_AUTO and *found = 0 (1 = bucket, 2 = matrix). Lookup
+ * needs A->max_row_nnz (or nnz = 0) and makes no HIP call.
+ * segscan_autotune_spmm times the built-in AUTO choice and a fixed candidate list of
+ * variants and knobs on the caller's data (C is overwritten; reps timed calls each, median,
+ * HIP events on `stream` with a short spin kernel queued before each start event so the
+ * host's issue cost overlaps it), returns the fastest and, with a table, records it under
+ * the matrix's key. It synchronises `stream`; candidates whose workspace exceeds
+ * workspace_bytes are skipped (segscan_autotune_workspace_size: the largest any
+ * candidate needs). A->max_row_nnz = 0 is read back from rowptr. */
+typedef struct segscan_tune_table* segscan_tune_table_t;
+const char* segscan_tune_prefix(void);
+segscan_status_t segscan_tune_table_create(segscan_tune_table_t* table);
+segscan_status_t segscan_tune_table_destroy(segscan_tune_table_t table);
+segscan_status_t segscan_tune_table_load(segscan_tune_table_t table, const char* path, int64_t* loaded,
+                                         int64_t* dropped);
+segscan_status_t segscan_tune_table_save(segscan_tune_table_t table, const char* path);
+segscan_status_t segscan_tune_table_lookup(segscan_tune_table_t table, const segscan_csr_t* A, int64_t N,
+                                           segscan_dtype_t b_dtype, segscan_spmm_variant_t* variant,
+                                           segscan_knobs_t* knobs, int32_t* found);
+segscan_status_t segscan_autotune_workspace_size(const segscan_csr_t* A, int64_t N, segscan_dtype_t b_dtype,
+                                                 size_t* bytes);
+segscan_status_t segscan_autotune_spmm(segscan_tune_table_t table, const segscan_csr_t* A, int64_t N,
+                                       const void* B, int64_t ldb, segscan_dtype_t b_dtype, float* C, int64_t ldc,
+                                       int32_t reps, void* workspace, size_t workspace_bytes, void* stream,
+                                       segscan_spmm_variant_t* variant, segscan_knobs_t* knobs, float* ms);
+
 /* ---- device-side input checks (optional; the compute calls trust their inputs) ---------
  * Writes into *flags_dev (device int32) a bit set: 1 rowptr not monotone or not
  * spanning [0, nnz], 2 column index out of [0, ncol), 4 columns not strictly increasing in a row. */

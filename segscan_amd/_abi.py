@@ -125,6 +125,17 @@ _SIGNATURES = {
     "segscan_batch_plan_run": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
                                          c_int64, c_void_p, c_int64, c_void_p, c_void_p]),
     "segscan_batch_plan_destroy": (c_int32, [c_void_p]),
+    "segscan_tune_prefix": (c_char_p, []),
+    "segscan_tune_table_create": (c_int32, [POINTER(c_void_p)]),
+    "segscan_tune_table_destroy": (c_int32, [c_void_p]),
+    "segscan_tune_table_load": (c_int32, [c_void_p, c_char_p, POINTER(c_int64), POINTER(c_int64)]),
+    "segscan_tune_table_save": (c_int32, [c_void_p, c_char_p]),
+    "segscan_tune_table_lookup": (c_int32, [c_void_p, POINTER(CsrDesc), c_int64, c_int32, POINTER(c_int32),
+                                            POINTER(Knobs), POINTER(c_int32)]),
+    "segscan_autotune_workspace_size": (c_int32, [POINTER(CsrDesc), c_int64, c_int32, POINTER(c_size_t)]),
+    "segscan_autotune_spmm": (c_int32, [c_void_p, POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p,
+                                        c_int64, c_int32, c_void_p, c_size_t, c_void_p, POINTER(c_int32),
+                                        POINTER(Knobs), POINTER(ctypes.c_float)]),
     "segscan_spmm_csr_noalloc": (c_int32, [POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p, c_int64,
                                            c_int32, POINTER(Knobs), c_uint32, c_void_p]),
     "segscan_segment_coo_describe": (c_int32, [c_int64, c_int64, c_int64, c_int32, c_int32, POINTER(Knobs), POINTER(LaunchInfo)]),

@@ -9,7 +9,7 @@ reference's knob points mapped to wave64 plus the gfx950-specific ones (owner
 write policy, long chunks, 16-lane groups over a 256-byte row).
 
 Two kinds of table entries:
-  bucket  "<arch>|<kind>|<dtype>|N<n>|skew<c>|<size>": fitted from a sweep
+  bucket  "<arch>|rocm<major>.<minor>|<kind>|<dtype>|N<n>|skew<c>|<size>": fitted from a sweep
           (``fit_table``): the configuration with the lowest geometric mean of
           time / per-matrix best over the sweep's matrices in the bucket;
   matrix  bucket key + "|m<rows>x<cols>:<nnz>:<max row>": the winner of a
@@ -20,9 +20,16 @@ capped at 3; size = "s" below 2^20 nonzeros, else "l". Knobs may hold
 "waves_per_cu" instead of a block factor; ``resolve_knobs`` turns it into a
 grid for the matrix at hand. Timing uses HIP events on torch's current stream
 with the host-side issue cost hidden (``time_call``).
+
+Keys start with the library's segscan_tune_prefix() (architecture and ROCm
+release); ``load_table`` drops entries of another prefix, so a table tuned under
+an older ROCm is re-tuned rather than misapplied. The same JSON files are read
+and written by the C ABI's tuning tables (``NativeTable``, segscan_tune_table_*),
+whose autotuner (segscan_autotune_spmm) times the same candidate list in C++.
 """
 from __future__ import annotations
 
+import ctypes
 import json
 import math
 from pathlib import Path
@@ -69,13 +76,19 @@ def n_bucket(N: int) -> int:
 _DT = {torch.float32: "f32", torch.bfloat16: "bf16", torch.float16: "f16"}
 
 
-def table_key(kind: str, dtype, N: int, stats: dict, arch: str = "gfx950") -> str:
+def tune_prefix() -> str:
+    """Architecture and ROCm release every key starts with, e.g. "gfx950|rocm7.2"."""
+    return _abi.load().segscan_tune_prefix().decode()
+
+
+def table_key(kind: str, dtype, N: int, stats: dict, prefix: str | None = None) -> str:
     dt = _DT.get(dtype, dtype)
-    return f"{arch}|{kind}|{dt}|N{n_bucket(N)}|skew{skew_class(stats)}|{size_class(stats)}"
+    pre = tune_prefix() if prefix is None else prefix
+    return f"{pre}|{kind}|{dt}|N{n_bucket(N)}|skew{skew_class(stats)}|{size_class(stats)}"
 
 
-def matrix_key(kind: str, dtype, N: int, stats: dict, ncol: int, arch: str = "gfx950") -> str:
-    return table_key(kind, dtype, N, stats, arch) + f"|m{stats['rows']}x{ncol}:{stats['nnz']}:{stats['max']}"
+def matrix_key(kind: str, dtype, N: int, stats: dict, ncol: int, prefix: str | None = None) -> str:
+    return table_key(kind, dtype, N, stats, prefix) + f"|m{stats['rows']}x{ncol}:{stats['nnz']}:{stats['max']}"
 
 
 def resolve_knobs(knobs: dict | None, stats: dict) -> dict | None:
@@ -312,11 +325,20 @@ def tune_segment(src: torch.Tensor, index: torch.Tensor, dst_len: int, candidate
 
 # ---- table ---------------------------------------------------------------------------------
 
-def load_table(path: Path | str = DEFAULT_TABLE) -> dict:
+def load_table(path: Path | str = DEFAULT_TABLE, stale: list | None = None) -> dict:
+    """The table's entries for this architecture and ROCm release; keys of any other
+    prefix (older releases, other GPUs) are left out and appended to ``stale``."""
     p = Path(path)
     if not p.exists():
         return {}
-    return json.loads(p.read_text())
+    pre = tune_prefix() + "|"
+    table = {}
+    for k, v in json.loads(p.read_text()).items():
+        if k.startswith(pre):
+            table[k] = v
+        elif stale is not None:
+            stale.append(k)
+    return table
 
 
 def save_table(table: dict, path: Path | str = DEFAULT_TABLE) -> None:
@@ -332,6 +354,64 @@ def lookup(table: dict, kind: str, dtype, N: int, stats: dict, ncol: int | None 
         if entry is not None:
             return entry["variant"], resolve_knobs(entry.get("knobs"), stats), key
     return None
+
+
+class NativeTable:
+    """The C ABI's tuning table (segscan_tune_table_*): load / save the JSON files this
+    module writes, look up a matrix, and run the C++ autotuner (segscan_autotune_spmm)
+    on device data, recording the winner under the matrix's key."""
+
+    def __init__(self, path: Path | str | None = None):
+        lib = _abi.load()
+        h = ctypes.c_void_p()
+        _abi.check(lib.segscan_tune_table_create(ctypes.byref(h)), "tune_table_create")
+        self._h = h
+        self.loaded = self.dropped = 0
+        if path is not None:
+            self.load(path)
+
+    def load(self, path) -> None:
+        nl, nd = ctypes.c_int64(), ctypes.c_int64()
+        _abi.check(_abi.load().segscan_tune_table_load(self._h, str(path).encode(), ctypes.byref(nl), ctypes.byref(nd)),
+                   "tune_table_load")
+        self.loaded, self.dropped = nl.value, nd.value
+
+    def save(self, path) -> None:
+        _abi.check(_abi.load().segscan_tune_table_save(self._h, str(path).encode()), "tune_table_save")
+
+    def lookup(self, A: ops.CsrMatrix, N: int, dtype=torch.float32):
+        """(variant id, knobs dict, found: 0 none / 1 bucket / 2 matrix)."""
+        v, k, f = ctypes.c_int32(), _abi.Knobs(), ctypes.c_int32()
+        _abi.check(_abi.load().segscan_tune_table_lookup(self._h, ctypes.byref(A.desc()), N, ops._DTYPES[dtype],
+                                                         ctypes.byref(v), ctypes.byref(k), ctypes.byref(f)),
+                   "tune_table_lookup")
+        return v.value, k.as_dict(), f.value
+
+    def autotune(self, A: ops.CsrMatrix, B: torch.Tensor, reps: int = 7, record: bool = True):
+        """Times the candidates on (A, B) in C++; returns (variant id, knobs dict, ms)."""
+        lib = _abi.load()
+        N = B.size(1)
+        desc = A.desc()
+        ws_bytes = ctypes.c_size_t()
+        _abi.check(lib.segscan_autotune_workspace_size(ctypes.byref(desc), N, ops._dtype_code(B),
+                                                       ctypes.byref(ws_bytes)), "autotune_workspace_size")
+        ws = torch.empty(max(1, ws_bytes.value), dtype=torch.uint8, device=B.device)
+        C = torch.empty((A.nrow, N), dtype=torch.float32, device=B.device)
+        v, k, ms = ctypes.c_int32(), _abi.Knobs(), ctypes.c_float()
+        _abi.check(lib.segscan_autotune_spmm(self._h if record else None, ctypes.byref(desc), N, ops._ptr(B),
+                                             ops._leading_dim(B, "B"), ops._dtype_code(B), ops._ptr(C), N, reps,
+                                             ops._ptr(ws), ws.numel(), ops._stream(None), ctypes.byref(v),
+                                             ctypes.byref(k), ctypes.byref(ms)), "autotune_spmm")
+        return v.value, k.as_dict(), ms.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _abi.load().segscan_tune_table_destroy(h)
+            except Exception:
+                pass
+            self._h = None
 
 
 def fit_table(rows: list[dict], kind: str = "spmm", exclude: str | None = None, margin: float = 1.03,

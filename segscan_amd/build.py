@@ -30,6 +30,7 @@ SOURCES = [
     "segscan_abi.cpp",
     "noalloc.hip",
     "batch_kernels.hip",
+    "tune_table.hip",
     "mtx_loader.cpp",
 ]
 

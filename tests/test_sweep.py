@@ -156,7 +156,7 @@ def test_fit_table_and_leave_one_out():
     from segscan_amd import autotune
     rows = _bucket_rows()
     table = autotune.fit_table(rows)
-    key = "gfx950|spmm|f32|N64|skew0|s"
+    key = autotune.tune_prefix() + "|spmm|f32|N64|skew0|s"
     assert list(table) == [key]
     e = table[key]  # gmeans: a1 2.5^(1/4) = 1.257, a2 3.6^(1/4) = 1.377
     assert e["action"] == "a1" and e["variant"] == "ebsr" and e["knobs"]["waves_per_cu"] == 16
