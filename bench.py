@@ -83,8 +83,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-tuned", action="store_true", help="skip the comparison legs (for profiling)")
     ap.add_argument("--dump", type=str, default=None, help="write tuning results to this JSON file")
-    ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "segment_f32", "segment_bf16", "batched_graph",
-                                           "batched_graph_kernels"],
+    ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "fwd_bwd_csr_values", "segment_f32", "segment_bf16",
+                                           "batched_graph", "batched_graph_kernels"],
                     default=None,
                     help="run one comparison leg alone (for per-kernel profiles) and print its timing")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -108,7 +108,8 @@ def copy_calibration():
 def only_leg(args):
     """One leg of the bench alone, `warmup` + `steps` calls, for rocprofv3 summaries of a
     single kernel configuration: the tuned bf16-B SpMM, the fused backward (full: dB and
-    dvals, values through perm), segment_coo AUTO over the power-law index (E = 4.19M,
+    dvals; values and dvals in the transpose's order, or for fwd_bwd_csr_values in A's
+    order through perm), segment_coo AUTO over the power-law index (E = 4.19M,
     N = 64), or one replay of the batched step's hipGraph (256 uneven graphs, 2.39M nodes,
     9.58M nnz, bf16 features, N = 64: forward SpMM, per-graph sum pooling and the backward
     for dH = dP[graph]) as the batch plan's single launch (batched_graph) or kernel by
@@ -141,13 +142,19 @@ def only_leg(args):
         C = torch.empty((m.nrow, N), dtype=torch.float32, device=dev)
         ws = ops.spmm_workspace(A, N, torch.bfloat16, v, k)
         fn = lambda: ops.spmm(A, Bbf, variant=v, knobs=k, out=C, workspace=ws)  # noqa: E731
-    elif args.only_leg == "fwd_bwd":
+    elif args.only_leg in ("fwd_bwd", "fwd_bwd_csr_values"):
         dC = torch.from_numpy(rng.standard_normal((m.nrow, N)).astype(np.float32)).to(dev)
         T = ops.csr_transpose(A)
         dB = torch.empty_like(B32)
         dvals = torch.empty(m.nnz, dtype=torch.float32, device=dev)
         ws = ops.spmm_backward_workspace(T, N, torch.float32)
-        fn = lambda: ops.spmm_backward(T, A.vals, dC, B32, dB=dB, dvals=dvals, workspace=ws)  # noqa: E731
+        if args.only_leg == "fwd_bwd":
+            # learnable values kept in the transpose's order: read and written coalesced
+            vals_t = A.vals.index_select(0, T.perm.long())
+            fn = lambda: ops.spmm_backward(T, None, dC, B32, dB=dB, dvals=dvals, workspace=ws,  # noqa: E731
+                                           vals_t=vals_t, dvals_at_order=True)
+        else:
+            fn = lambda: ops.spmm_backward(T, A.vals, dC, B32, dB=dB, dvals=dvals, workspace=ws)  # noqa: E731
     else:
         dt = torch.float32 if args.only_leg == "segment_f32" else torch.bfloat16
         idx = torch.from_numpy(synth.segment_index_from_degrees(m.degrees())).to(dev)
@@ -305,7 +312,20 @@ def main():
         ws_f = ops.spmm_workspace(A, N, torch.float32, v32, k32)
         ws_b = ops.spmm_backward_workspace(T, N, torch.float32)
 
+        # learnable values kept in the transpose's order (the autograd layer's
+        # values_order="transpose"): the forward reads them through the inverse permutation,
+        # the backward reads them and writes dvals coalesced
+        iperm = ops.invert_permutation(T.perm)
+        vals_t = A.vals.index_select(0, T.perm.long())
+        A_at = A.with_vals_map(vals_t, iperm)
+
         def fwd_bwd():
+            ops.spmm(A_at, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
+            ops.spmm_backward(T, None, dC, B32, dB=dB, dvals=dvals, workspace=ws_b, vals_t=vals_t,
+                              dvals_at_order=True)
+
+        def fwd_bwd_csr_values():
+            # values in A's order: the backward reads them and writes dvals through perm
             ops.spmm(A, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
             ops.spmm_backward(T, A.vals, dC, B32, dB=dB, dvals=dvals, workspace=ws_b)
 
@@ -325,6 +345,7 @@ def main():
             ops.spmm_backward(Tn, A.vals, dC, B32, dB=dB, dvals=dvals, workspace=ws_b)
 
         fb_ms, _ = timed(fwd_bwd, 10, 3, kernel_events=False)
+        fbc_ms, _ = timed(fwd_bwd_csr_values, 10, 3, kernel_events=False)
         fbs_ms, _ = timed(fwd_bwd_separate, 10, 3, kernel_events=False)
         fbf_ms, _ = timed(fwd_bwd_fixed_vals, 10, 3, kernel_events=False)
         fbt_ms, _ = timed(fwd_bwd_with_transpose, 5, 2, kernel_events=False)
@@ -356,6 +377,7 @@ def main():
             "hipsparse_f32_ms": vendor,
             "hipsparse_best_over_tuned_f32": None if best_vendor is None else round(best_vendor / ms_step, 3),
             "fwd_bwd_ms": round(fb_ms, 4),
+            "fwd_bwd_ms_csr_order_values": round(fbc_ms, 4),
             "fwd_bwd_ms_separate_kernels": round(fbs_ms, 4),
             "fwd_bwd_ms_fixed_vals": round(fbf_ms, 4),
             "fwd_bwd_ms_incl_transpose": round(fbt_ms, 4),

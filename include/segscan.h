@@ -143,6 +143,10 @@ typedef struct segscan_csr {
   const float* vals;     /* [nnz], device, or NULL */
   int64_t max_row_nnz;   /* longest row, or 0 = unknown; guides SEGSCAN_SPMM_AUTO only
                             (0 <= max_row_nnz <= nnz). Zero-initialise the struct. */
+  const int32_t* vals_map; /* NULL, or [nnz] device: entry p's value is vals[vals_map[p]], for
+                              values kept in another order (e.g. the transpose's, whose
+                              backward then reads and writes them coalesced; vals_map is
+                              the inverse of segscan_csr_transpose's perm). SpMM only. */
 } segscan_csr_t;
 
 /* Flags. Without SEGSCAN_ACCUMULATE the call computes C = A.B (resp. dst =
@@ -150,6 +154,10 @@ typedef struct segscan_csr {
  * zero C (the reference requires a pre-zeroed C, check.cuh:71). With it the
  * call computes C += A.B. */
 #define SEGSCAN_ACCUMULATE 1u
+/* segscan_spmm_backward_csr(_rowmap): dvals is written in At's order (dvals[q] for At entry
+ * q) instead of A's (dvals[perm[q]]): coalesced stores, for values kept in At's order. With
+ * At.vals set (or all-ones values) the call then reads no perm at all. */
+#define SEGSCAN_DVALS_AT_ORDER 2u
 
 /* Launch geometry a call would use; filled by the *_describe functions (host
  * only, no HIP call), for tests and for the autotuner's bookkeeping. */
@@ -277,6 +285,9 @@ segscan_status_t segscan_spmm_backward_csr_rowmap(const segscan_csr_t* At, const
 segscan_status_t segscan_segment_gather(const float* ddst, int64_t ld_ddst, const int32_t* index, int64_t E,
                                         int64_t N, const int32_t* counts, float* dsrc, int64_t ld_dsrc,
                                         void* stream);
+/* inv[perm[i]] = i for i < n (perm a permutation of [0, n)): the vals_map that reads values
+ * kept in the transpose's order from A's entries. */
+segscan_status_t segscan_invert_permutation(const int32_t* perm, int64_t n, int32_t* inv, void* stream);
 /* counts[s] = #{e : index[e] == s}, s < dst_len (sorted index). */
 segscan_status_t segscan_segment_counts(const int32_t* index, int64_t E, int64_t dst_len, int32_t* counts,
                                         void* stream);

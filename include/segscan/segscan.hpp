@@ -93,7 +93,7 @@ struct SpMatCsrDescr_t {
     sp_data.upload();
   }
   segscan_csr_t desc() const {
-    segscan_csr_t d;
+    segscan_csr_t d{};
     d.nrow = nrow;
     d.ncol = ncol;
     d.nnz = nnz;

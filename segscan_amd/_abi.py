@@ -25,6 +25,7 @@ SEG_PR, SEG_PR_NEW, SEG_SR, SEG_NOSHMEM_SR, SEG_LESSATOM_SR, SEG_SORTED_SR, SEG_
 # segscan_write_policy_t
 WRITE_DEFAULT, WRITE_ATOMIC, WRITE_OWNER, WRITE_LESSATOM = 0, 1, 2, 3
 ACCUMULATE = 1
+DVALS_AT_ORDER = 2
 # segscan_mtx_read flags
 MTX_KEEP_VALUES, MTX_REFERENCE_QUIRKS = 1, 2
 
@@ -44,7 +45,7 @@ class CsrDesc(ctypes.Structure):
     _fields_ = [
         ("nrow", c_int64), ("ncol", c_int64), ("nnz", c_int64),
         ("rowptr", c_void_p), ("colind", c_void_p), ("vals", c_void_p),
-        ("max_row_nnz", c_int64),
+        ("max_row_nnz", c_int64), ("vals_map", c_void_p),
     ]
 
 
@@ -136,6 +137,7 @@ _SIGNATURES = {
     "segscan_autotune_spmm": (c_int32, [c_void_p, POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p,
                                         c_int64, c_int32, c_void_p, c_size_t, c_void_p, POINTER(c_int32),
                                         POINTER(Knobs), POINTER(ctypes.c_float)]),
+    "segscan_invert_permutation": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
     "segscan_spmm_csr_noalloc": (c_int32, [POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p, c_int64,
                                            c_int32, POINTER(Knobs), c_uint32, c_void_p]),
     "segscan_segment_coo_describe": (c_int32, [c_int64, c_int64, c_int64, c_int32, c_int32, POINTER(Knobs), POINTER(LaunchInfo)]),

@@ -2,7 +2,10 @@
 over the same C ABI as the forward (libsegscan_hip.so); nothing here runs on the
 CPU.
 
-SpMM  C = A(vals) . B, A fixed pattern (rowptr, colind), vals and B learnable:
+SpMM  C = A(vals) . B, A fixed pattern (rowptr, colind), vals and B learnable
+(vals in CSR order, or with values_order="transpose" in the transpose's order: the
+forward reads them through the inverse permutation and the fused backward reads them
+and writes their gradient coalesced, with no random access per entry):
     dB    = A^T . dC     CSR -> CSC transpose on the device (cached per pattern)
     dvals = SDDMM(dC, B) on A's pattern: dvals[p] = <dC[row(p)], B[colind[p]]>
     When dvals is wanted, both come from one fused walk over the transpose (one
@@ -24,8 +27,11 @@ class SpmmPattern:
     """A CSR pattern with its device transpose computed once."""
 
     def __init__(self, A: ops.CsrMatrix, variant="auto", knobs=None, variant_t="auto", knobs_t=None,
-                 fused: bool = True, knobs_bwd=None):
+                 fused: bool = True, knobs_bwd=None, values_order: str = "csr"):
+        if values_order not in ("csr", "transpose"):
+            raise ValueError("values_order must be 'csr' or 'transpose'")
         self.A = A
+        self.values_order = values_order
         T = ops.csr_transpose(A)
         self.T = ops.CscMatrix(T.t.measured(), T.perm)  # longest column guides the "auto" plan of A^T
         self.variant, self.knobs = variant, knobs
@@ -34,6 +40,8 @@ class SpmmPattern:
         self.knobs_bwd = knobs_bwd  # fused backward shape (e.g. from autotune.tune_backward); None = built-in
         self._fused = {}
         self._vals_t, self._vals_t_key = None, None
+        # values kept in the transpose's order: the forward reads entry p's value at iperm[p]
+        self.iperm = ops.invert_permutation(T.perm) if values_order == "transpose" else None
 
     def fused_ok(self, N: int) -> bool:
         """Whether the fused backward covers N columns (one group of lanes per dense row)."""
@@ -50,7 +58,16 @@ class SpmmPattern:
         return self._fused[N]
 
     def with_vals(self, vals: torch.Tensor) -> ops.CsrMatrix:
+        """A with vals in this pattern's value order."""
+        if self.values_order == "transpose":
+            return self.A.with_vals_map(vals, self.iperm)
         return self.A.with_vals(vals)
+
+    def to_pattern_order(self, vals_csr: torch.Tensor) -> torch.Tensor:
+        """CSR-order values in this pattern's value order (set-up of a parameter)."""
+        if self.values_order == "transpose":
+            return vals_csr.index_select(0, self.T.perm.long())
+        return vals_csr
 
     def vals_in_t_order(self, vals: torch.Tensor) -> torch.Tensor:
         """vals permuted into the transpose's order, cached while vals is unchanged
@@ -62,6 +79,8 @@ class SpmmPattern:
         return self._vals_t
 
     def transposed_with_vals(self, vals: torch.Tensor) -> ops.CsrMatrix:
+        if self.values_order == "transpose":
+            return self.T.t.with_vals(vals)
         return self.T.t.with_vals(self.vals_in_t_order(vals))
 
 
@@ -82,21 +101,27 @@ class SpmmFunction(torch.autograd.Function):
         dvals = dB = None
         want_b, want_v = ctx.needs_input_grad[1], ctx.needs_input_grad[0]
         if want_v and pattern.fused_ok(dC.size(1)):
-            # learnable values: both gradients from one walk over the transpose, the values
-            # read through the permutation inside the kernel
+            # learnable values: both gradients from one walk over the transpose
             v32 = vals.detach().float().contiguous()
             # a 16-bit B is read as stored (fp32 dC, fp32 accumulation)
             Bs = B.detach().contiguous()
             if Bs.dtype not in (torch.bfloat16, torch.float16):
                 Bs = Bs.float()
-            dB, dvals = ops.spmm_backward(pattern.T, v32, dC, Bs, want_dB=want_b, knobs=pattern.knobs_bwd)
+            if pattern.values_order == "transpose":
+                # values and their gradient in the walk's own order: no perm at all
+                dB, dvals = ops.spmm_backward(pattern.T, None, dC, Bs, want_dB=want_b, knobs=pattern.knobs_bwd,
+                                              vals_t=v32, dvals_at_order=True)
+            else:
+                # values read through the permutation inside the kernel
+                dB, dvals = ops.spmm_backward(pattern.T, v32, dC, Bs, want_dB=want_b, knobs=pattern.knobs_bwd)
             return dvals.to(vals.dtype), (None if dB is None else dB.to(B.dtype)), None
         # dB alone (fixed values): the SpMM kernels on the transpose, values in its order (cached)
         if ctx.needs_input_grad[1]:
             At = pattern.transposed_with_vals(vals.detach().float())
             dB = ops.spmm(At, dC, variant=pattern.variant_t, knobs=pattern.knobs_t).to(B.dtype)
         if ctx.needs_input_grad[0]:
-            dvals = ops.sddmm(pattern.A, dC, B.detach().float().contiguous()).to(vals.dtype)
+            dvals = ops.sddmm(pattern.A, dC, B.detach().float().contiguous())
+            dvals = pattern.to_pattern_order(dvals).to(vals.dtype)
         return dvals, dB, None
 
 

@@ -659,6 +659,7 @@ segscan_status_t segscan_batch_plan_create(const segscan_csr_t* A, const int64_t
   if (A->nnz < 0 || A->nnz > kMaxI32) return einval("nnz out of range");
   if (A->nrow > 0 && !A->rowptr) return einval("rowptr is NULL");
   if (A->nnz > 0 && !A->colind) return einval("colind is NULL");
+  if (A->vals_map) return set_error(SEGSCAN_EUNSUPPORTED, "the batch plan reads values in A's order (vals_map NULL)");
   if (!graph_offsets) return einval("graph_offsets is NULL");
   if (n_graphs < 1 || n_graphs >= kMaxI32) return einval("n_graphs out of range");
   if (graph_offsets[0] != 0 || graph_offsets[n_graphs] != A->nrow)

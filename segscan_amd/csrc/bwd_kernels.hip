@@ -53,6 +53,7 @@ __device__ __forceinline__ void bwd_walk(const BwdParams& p, int32_t* bwd_smem) 
   const bool want_db = p.dB != nullptr;
   const bool want_dv = p.dvals != nullptr;
   const bool acc_mode = p.accumulate != 0;
+  const bool need_perm = (!p.vals_t && p.vals) || (want_dv && !p.dvals_at_order);
 
   for (int64_t c = ggid;; c += p.total_groups) {
     if (__ballot(c < p.nchunks) == 0) break;
@@ -118,7 +119,8 @@ __device__ __forceinline__ void bwd_walk(const BwdParams& p, int32_t* bwd_smem) 
       const int64_t en = c0 + off_n + g_lane;
       pk = vn ? p.colind[en] : 0;
       if (p.dc_map != nullptr && vn) pk = p.dc_map[pk];  // dC row of A's row pk
-      ppos = vn ? p.perm[en] : 0;
+      // A position: for values read through perm, and for dvals in A's order
+      ppos = (vn && need_perm) ? p.perm[en] : 0;
       if (p.vals_t) pvt = vn ? p.vals_t[en] : 1.0f;
       if (off_n < clen) {
         const int jj = r_hint + 1 + g_lane;
@@ -141,7 +143,7 @@ __device__ __forceinline__ void bwd_walk(const BwdParams& p, int32_t* bwd_smem) 
       const float myv = p.vals_t ? pvt : ((valid && p.vals) ? p.vals[pos] : 1.0f);
       s_k[L] = pk;
       s_r[L] = valid ? row : INT32_MAX;
-      s_p[L] = pos;
+      s_p[L] = p.dvals_at_order ? static_cast<int>(c0 + off + g_lane) : pos;  // where dvals goes
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -22,6 +22,7 @@ struct PrParams {
   const int32_t* rowptr;   // SpMM only
   const int32_t* colind;   // SpMM only
   const float* vals;       // SpMM only, nullable
+  const int32_t* vmap;     // nullable: entry e's value is vals[vmap[e]]
   const int32_t* index;    // segment only
   const void* B;           // dense operand (B or src)
   int64_t ldb;
@@ -46,6 +47,7 @@ struct SrParams {
   const int32_t* rowptr;
   const int32_t* colind;
   const float* vals;
+  const int32_t* vmap;        // nullable: entry e's value is vals[vmap[e]]
   const int32_t* index;
   const void* B;
   int64_t ldb;
@@ -107,6 +109,7 @@ struct BwdParams {
   int32_t* carry_row;      // [nchunks] row of the chunk's carry or -1
   float* carry_val;        // [nchunks x N]
   int accumulate;          // dB += instead of dB =
+  int dvals_at_order;      // dvals[q] (At's order) instead of dvals[perm[q]]
 };
 
 // Row-balanced sequential reduction (rbsr): lanes [rows][column vectors].
@@ -116,6 +119,7 @@ struct RbsrParams {
   const int32_t* rowptr;
   const int32_t* colind;
   const float* vals;
+  const int32_t* vmap;  // nullable: entry e's value is vals[vmap[e]]
   const void* B;
   int64_t ldb;
   float* C;
@@ -205,6 +209,7 @@ hipError_t launch_segment_gather(const float* ddst, int64_t ld_ddst, const int32
                                  const int32_t* counts, float* dsrc, int64_t ld_dsrc, hipStream_t s);
 hipError_t launch_segment_counts(const int32_t* index, int64_t E, int64_t dst_len, int32_t* counts,
                                  hipStream_t s);
+hipError_t launch_invert_permutation(const int32_t* perm, int64_t n, int32_t* inv, hipStream_t s);
 hipError_t launch_check_csr(const int32_t* rowptr, const int32_t* colind, int64_t M, int64_t K, int64_t nnz,
                             int32_t* flags, hipStream_t s);
 hipError_t launch_check_index(const int32_t* index, int64_t E, int64_t dst_len, int32_t* flags, hipStream_t s);

@@ -540,6 +540,22 @@ hipError_t launch_segment_counts(const int32_t* index, int64_t E, int64_t dst_le
   return hipGetLastError();
 }
 
+// ---- inverse permutation: inv[perm[i]] = i ----
+
+__global__ void invert_permutation_kernel(const int32_t* __restrict__ perm, int64_t n, int32_t* __restrict__ inv) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int32_t j = perm[i];
+    if (j >= 0 && j < n) inv[j] = static_cast<int32_t>(i);
+  }
+}
+
+hipError_t launch_invert_permutation(const int32_t* perm, int64_t n, int32_t* inv, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  invert_permutation_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(perm, n, inv);
+  return hipGetLastError();
+}
+
 // ---- device-side input checks ----
 
 __global__ void check_csr_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colind, int64_t M,

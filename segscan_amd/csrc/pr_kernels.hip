@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(1024) pr_eb_kernel(PrParams p) {
         v = 1.0f;
       } else {
         k = p.colind[e];
-        v = p.vals ? p.vals[e] : 1.0f;
+        v = p.vals ? p.vals[p.vmap ? p.vmap[e] : e] : 1.0f;
       }
       load_cols_any<VT, CF>(B + k * p.ldb + col0, nv, c);
 #pragma unroll
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(1024) pr_rb_kernel(PrParams p) {
     for (int i = 0; i < CF; ++i) c[i] = 0.0f;
     for (int jj = start + ln.t; jj < end; jj += p.X) {
       const int64_t k = p.colind[jj];
-      const float v = p.vals ? p.vals[jj] : 1.0f;
+      const float v = p.vals ? p.vals[p.vmap ? p.vmap[jj] : jj] : 1.0f;
       float b[CF];
       load_cols_any<VT, CF>(B + k * p.ldb + col0, nv, b);
 #pragma unroll

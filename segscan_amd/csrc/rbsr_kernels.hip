@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(1024) rbsr_kernel(RbsrParams p) {
 #pragma unroll
       for (int u = 0; u < RB; ++u) {
         k[u] = p.colind[q + u];
-        v[u] = p.vals ? p.vals[q + u] : 1.0f;
+        v[u] = p.vals ? p.vals[p.vmap ? p.vmap[q + u] : q + u] : 1.0f;
       }
       RawCols<VT, CF> raw[RB];
       if (full) {
@@ -65,8 +65,8 @@ __global__ void __launch_bounds__(1024) rbsr_kernel(RbsrParams p) {
     for (; q + 1 < end; q += 2) {
       const int64_t k0 = p.colind[q];
       const int64_t k1 = p.colind[q + 1];
-      const float v0 = p.vals ? p.vals[q] : 1.0f;
-      const float v1 = p.vals ? p.vals[q + 1] : 1.0f;
+      const float v0 = p.vals ? p.vals[p.vmap ? p.vmap[q] : q] : 1.0f;
+      const float v1 = p.vals ? p.vals[p.vmap ? p.vmap[q + 1] : q + 1] : 1.0f;
       float b0[CF], b1[CF];
       load_cols_any<VT, CF>(B + k0 * p.ldb + col0, nv, b0);
       load_cols_any<VT, CF>(B + k1 * p.ldb + col0, nv, b1);
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(1024) rbsr_kernel(RbsrParams p) {
     }
     if (q < end) {
       const int64_t k0 = p.colind[q];
-      const float v0 = p.vals ? p.vals[q] : 1.0f;
+      const float v0 = p.vals ? p.vals[p.vmap ? p.vmap[q] : q] : 1.0f;
       float b0[CF];
       load_cols_any<VT, CF>(B + k0 * p.ldb + col0, nv, b0);
 #pragma unroll

@@ -388,6 +388,7 @@ segscan_status_t validate_csr(const segscan_csr_t* A, bool need_pointers) {
   if (A->ncol == 0 && A->nnz > 0) return fail(SEGSCAN_EINVAL, "nnz > 0 with no columns");
   if (A->max_row_nnz < 0 || A->max_row_nnz > A->nnz)
     return fail(SEGSCAN_EINVAL, "max_row_nnz %lld outside [0, nnz]", (long long)A->max_row_nnz);
+  if (A->vals_map && !A->vals) return fail(SEGSCAN_EINVAL, "vals_map given without vals");
   if (need_pointers) {
     if (A->nrow > 0 && !A->rowptr) return fail(SEGSCAN_EINVAL, "rowptr is NULL");
     if (A->nnz > 0 && !A->colind) return fail(SEGSCAN_EINVAL, "colind is NULL");
@@ -635,7 +636,7 @@ void fill_info(const SegPlan& pl, segscan_launch_info_t* info) {
 
 extern "C" {
 
-int32_t segscan_abi_version(void) { return 2; }
+int32_t segscan_abi_version(void) { return 3; }
 
 void segscan_set_timing_events(void* start, void* stop) {
   g_ev_start = static_cast<hipEvent_t>(start);
@@ -759,6 +760,7 @@ segscan_status_t spmm_csr_impl(const segscan_csr_t* A, int64_t N, const void* B,
       p.rowptr = A->rowptr;
       p.colind = A->colind;
       p.vals = A->vals;
+      p.vmap = A->vals_map;
       p.B = B;
       p.ldb = ldb;
       p.C = C;
@@ -789,6 +791,7 @@ segscan_status_t spmm_csr_impl(const segscan_csr_t* A, int64_t N, const void* B,
       p.rowptr = A->rowptr;
       p.colind = A->colind;
       p.vals = A->vals;
+      p.vmap = A->vals_map;
       p.B = B;
       p.ldb = ldb;
       p.C = C;
@@ -819,6 +822,7 @@ segscan_status_t spmm_csr_impl(const segscan_csr_t* A, int64_t N, const void* B,
       p.rowptr = A->rowptr;
       p.colind = A->colind;
       p.vals = A->vals;
+      p.vmap = A->vals_map;
       p.B = B;
       p.ldb = ldb;
       p.C = C;
@@ -989,6 +993,7 @@ segscan_status_t segscan_csr_transpose(const segscan_csr_t* A, int32_t* colptr, 
   if (st) return st;
   if (!colptr) return fail(SEGSCAN_EINVAL, "colptr is NULL");
   if (A->nnz > 0 && (!rowind || !perm)) return fail(SEGSCAN_EINVAL, "rowind / perm is NULL");
+  if (A->vals_map) return fail(SEGSCAN_EUNSUPPORTED, "transpose reads values in A's order (vals_map must be NULL)");
   size_t need = 0;
   segscan_csr_transpose_workspace_size(A, &need);
   if (!workspace) return fail(SEGSCAN_EWORKSPACE, "workspace of %zu bytes required", need);
@@ -1127,7 +1132,9 @@ static segscan_status_t spmm_backward_impl(const segscan_csr_t* At, const int32_
     return fail(SEGSCAN_EINVAL, "unknown dtype %d / %d", (int)dc_dtype, (int)b_dtype);
   if (dc_dtype != b_dtype && dc_dtype != SEGSCAN_F32)
     return fail(SEGSCAN_EUNSUPPORTED, "dC and B must share a dtype, or dC be fp32");
-  if (flags & ~SEGSCAN_ACCUMULATE) return fail(SEGSCAN_EINVAL, "unknown flags 0x%x", flags);
+  if (flags & ~(SEGSCAN_ACCUMULATE | SEGSCAN_DVALS_AT_ORDER)) return fail(SEGSCAN_EINVAL, "unknown flags 0x%x", flags);
+  if (At->vals_map) return fail(SEGSCAN_EINVAL, "At.vals_map must be NULL (At.vals are in At's order)");
+  const bool dvals_at = flags & SEGSCAN_DVALS_AT_ORDER;
   const int64_t K = At->nrow, nnz = At->nnz;
   const size_t es = dtype_size(dc_dtype), es_b = dtype_size(b_dtype);
   if (dB && K > 0 && N > 0) {
@@ -1135,7 +1142,8 @@ static segscan_status_t spmm_backward_impl(const segscan_csr_t* At, const int32_
     if (reinterpret_cast<uintptr_t>(dB) % 4) return fail(SEGSCAN_EINVAL, "dB is not 4-byte aligned");
   }
   if (nnz > 0) {
-    if (!perm) return fail(SEGSCAN_EINVAL, "perm is NULL");
+    // perm is read for values in A's order and for dvals in A's order
+    if (!perm && ((!At->vals && vals) || (dvals && !dvals_at))) return fail(SEGSCAN_EINVAL, "perm is NULL");
     if (N > 0) {
       if (!dC || !B) return fail(SEGSCAN_EINVAL, "dC / B is NULL");
       if (ld_dc < N || ld_b < N) return fail(SEGSCAN_EINVAL, "leading dimension < N");
@@ -1190,6 +1198,7 @@ static segscan_status_t spmm_backward_impl(const segscan_csr_t* At, const int32_
   p.carry_row = dB ? reinterpret_cast<int32_t*>(ws + pl.off_crow) : nullptr;
   p.carry_val = dB ? reinterpret_cast<float*>(ws + pl.off_cval) : nullptr;
   p.accumulate = accumulate;
+  p.dvals_at_order = dvals_at;
   Launch l;
   l.grid = dim3(static_cast<unsigned>(pl.grid));
   l.block = dim3(pl.tpb);
@@ -1241,6 +1250,13 @@ segscan_status_t segscan_segment_gather(const float* ddst, int64_t ld_ddst, cons
   return hip_status(launch_segment_gather(ddst, ld_ddst, index, E, static_cast<int>(N), counts, dsrc, ld_dsrc,
                                           static_cast<hipStream_t>(stream)),
                     "segment gather");
+}
+
+segscan_status_t segscan_invert_permutation(const int32_t* perm, int64_t n, int32_t* inv, void* stream) {
+  if (n < 0 || n > kMaxI32) return fail(SEGSCAN_EINVAL, "n out of range");
+  if (n == 0) return ok();
+  if (!perm || !inv) return fail(SEGSCAN_EINVAL, "NULL pointer");
+  return hip_status(launch_invert_permutation(perm, n, inv, static_cast<hipStream_t>(stream)), "invert permutation");
 }
 
 segscan_status_t segscan_segment_counts(const int32_t* index, int64_t E, int64_t dst_len, int32_t* counts,

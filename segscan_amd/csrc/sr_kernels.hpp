@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
         prow = vn ? p.index[en] : INT32_MAX;
       } else {
         pk = vn ? p.colind[en] : 0;
-        pv = vn ? (p.vals ? p.vals[en] : 1.0f) : 0.0f;
+        pv = vn ? (p.vals ? p.vals[p.vmap ? p.vmap[en] : en] : 1.0f) : 0.0f;
         if (off_n < clen) {
           const int jj = r_hint + 1 + g_lane;
           prp = (jj <= p.M) ? p.rowptr[jj] : INT32_MAX;

@@ -63,6 +63,7 @@ class CsrMatrix:
     colind: torch.Tensor
     vals: Optional[torch.Tensor] = None
     max_row_nnz: int = 0  # longest row, 0 = unknown; only guides variant="auto"
+    vals_map: Optional[torch.Tensor] = None  # int32 [nnz]: entry p's value is vals[vals_map[p]]
 
     @property
     def nnz(self) -> int:
@@ -79,12 +80,20 @@ class CsrMatrix:
                    max_row)
 
     def with_vals(self, vals: Optional[torch.Tensor]) -> "CsrMatrix":
+        """The same pattern with values in CSR order."""
         return CsrMatrix(self.nrow, self.ncol, self.rowptr, self.colind, vals, self.max_row_nnz)
+
+    def with_vals_map(self, vals: torch.Tensor, vals_map: torch.Tensor) -> "CsrMatrix":
+        """The same pattern with values kept in another order: entry p's value is
+        vals[vals_map[p]] (e.g. the transpose's order, vals_map = its inverse perm)."""
+        if vals_map.dtype != torch.int32 or vals_map.numel() != self.nnz or not vals_map.is_contiguous():
+            raise ValueError("vals_map must be contiguous int32 with nnz entries")
+        return CsrMatrix(self.nrow, self.ncol, self.rowptr, self.colind, vals, self.max_row_nnz, vals_map)
 
     def measured(self) -> "CsrMatrix":
         """The same matrix with max_row_nnz read from the device (one synchronisation)."""
         m = int(torch.diff(self.rowptr).max()) if self.nrow > 0 else 0
-        return CsrMatrix(self.nrow, self.ncol, self.rowptr, self.colind, self.vals, m)
+        return CsrMatrix(self.nrow, self.ncol, self.rowptr, self.colind, self.vals, m, self.vals_map)
 
     def desc(self) -> _abi.CsrDesc:
         d = _abi.CsrDesc()
@@ -93,6 +102,7 @@ class CsrMatrix:
         d.colind = self.colind.data_ptr() if self.colind.numel() else None
         d.vals = None if self.vals is None or self.vals.numel() == 0 else self.vals.data_ptr()
         d.max_row_nnz = self.max_row_nnz
+        d.vals_map = None if self.vals_map is None or self.vals_map.numel() == 0 else self.vals_map.data_ptr()
         return d
 
 
@@ -230,7 +240,8 @@ def spmm_backward_workspace(T: CscMatrix, N: int, dtype=torch.float32, knobs=Non
 def spmm_backward(T: CscMatrix, vals: Optional[torch.Tensor], dC: torch.Tensor, B: torch.Tensor, want_dB: bool = True,
                   want_dvals: bool = True, dB: Optional[torch.Tensor] = None, dvals: Optional[torch.Tensor] = None,
                   knobs=None, accumulate: bool = False, workspace: Optional[torch.Tensor] = None, stream=None,
-                  vals_t: Optional[torch.Tensor] = None, dc_map: Optional[torch.Tensor] = None):
+                  vals_t: Optional[torch.Tensor] = None, dc_map: Optional[torch.Tensor] = None,
+                  dvals_at_order: bool = False):
     """Fused backward of C = A @ B over T = csr_transpose(A), one gather of dC per entry:
     dB = A^T @ dC (K x N, fp32) and dvals[p] = <dC[row(p)], B[colind[p]]> (nnz, fp32, A's order).
     vals: A's values in A's order (None = ones), read through T.perm; vals_t: the same values
@@ -238,6 +249,8 @@ def spmm_backward(T: CscMatrix, vals: Optional[torch.Tensor], dC: torch.Tensor, 
     and used instead of vals. dc_map: int32, one entry per row of A; row r of A then reads
     dC[dc_map[r]] (dC may have any number of rows), e.g. dC = dP with dc_map = the pooling
     index, for the backward of a per-segment sum pooling without the gathered copy.
+    dvals_at_order: dvals in the transpose's order (coalesced stores), the gradient of
+    values kept in that order (with vals_t, the call reads no perm).
     Returns (dB or None, dvals or None)."""
     _require_device(dC, B)
     lib = _abi.load()
@@ -269,7 +282,8 @@ def spmm_backward(T: CscMatrix, vals: Optional[torch.Tensor], dC: torch.Tensor, 
         workspace = spmm_backward_workspace(T, N, dC.dtype, knobs, B.device) if dB is not None else None
     desc = T.t.with_vals(vals_t).desc()  # the At-order values, if any, travel in At.vals
     tail = (_ptr(B), _leading_dim(B, "B"), _dtype_code(B), _ptr(dB), _leading_dim(dB, "dB") if dB is not None else 0,
-            _ptr(dvals), kp, _abi.ACCUMULATE if accumulate else 0, _ptr(workspace),
+            _ptr(dvals), kp, (_abi.ACCUMULATE if accumulate else 0) | (_abi.DVALS_AT_ORDER if dvals_at_order else 0),
+            _ptr(workspace),
             0 if workspace is None else workspace.numel(), _stream(stream))
     if dc_map is not None:
         st = lib.segscan_spmm_backward_csr_rowmap(byref(desc), _ptr(T.perm), _ptr(vals), N, _ptr(dC),
@@ -279,6 +293,18 @@ def spmm_backward(T: CscMatrix, vals: Optional[torch.Tensor], dC: torch.Tensor, 
                                            _dtype_code(dC), *tail)
     _abi.check(st, "spmm_backward")
     return dB, dvals
+
+
+def invert_permutation(perm: torch.Tensor, stream=None) -> torch.Tensor:
+    """inv[perm[i]] = i (int32, on the device): the vals_map that reads values kept in the
+    transpose's order."""
+    _require_device(perm)
+    if perm.dtype != torch.int32 or not perm.is_contiguous():
+        raise ValueError("perm must be contiguous int32")
+    inv = torch.empty_like(perm)
+    _abi.check(_abi.load().segscan_invert_permutation(_ptr(perm), perm.numel(), _ptr(inv), _stream(stream)),
+               "invert_permutation")
+    return inv
 
 
 def segment_counts(index: torch.Tensor, dst_len: int, stream=None) -> torch.Tensor:

@@ -40,7 +40,7 @@ def test_library_exports_every_declared_symbol(lib):
 
 
 def test_version_and_status_strings(lib):
-    assert lib.segscan_abi_version() == 2
+    assert lib.segscan_abi_version() == 3
     for s in range(5):
         assert lib.segscan_status_string(s).decode().startswith("SEGSCAN_")
 
@@ -137,7 +137,7 @@ def test_fused_backward_validation_and_plan(lib):
     assert call(dC=None) == _abi.EINVAL
     assert call(ld_dc=4) == _abi.EINVAL
     assert call(ld_db=4) == _abi.EINVAL
-    assert call(flags=2) == _abi.EINVAL
+    assert call(flags=4) == _abi.EINVAL
     assert call(ws=None, ws_bytes=0) == _abi.EWORKSPACE
     assert call(dc_t=9) == _abi.EINVAL
     # a 16-bit dC needs a B of the same type; an fp32 dC takes any B
@@ -307,3 +307,27 @@ def test_batch_plan_validation(lib):
     assert lib.segscan_batch_plan_run(None, 0x1000, 16, None, 0, None, 0, 0x2000, 16, None, 0, None, None) == _abi.EINVAL
     assert lib.segscan_batch_plan_set_values(None, None, None) == _abi.EINVAL
     assert lib.segscan_batch_plan_destroy(None) == _abi.OK
+
+
+def test_values_order_validation(lib):
+    """vals_map needs values; the transpose and the backward's At refuse one; the backward
+    knows the At-order dvals flag; the inverse permutation checks its arguments."""
+    d = _desc(nrow=10, ncol=10, nnz=20)
+    d.vals_map = 0x3000
+    assert _spmm(lib, d) == _abi.EINVAL                      # a map without values
+    d.vals = 0x4000
+    assert _spmm(lib, d, variant=_abi.SPMM_EBSR, ws=None) in (_abi.EWORKSPACE, _abi.OK, _abi.EHIP)
+    ws = ctypes.c_size_t()
+    assert lib.segscan_csr_transpose_workspace_size(byref(d), byref(ws)) == _abi.OK
+    assert lib.segscan_csr_transpose(byref(d), 0x5000, 0x6000, 0x7000, None, 0x8000, ws.value, None) == \
+        _abi.EUNSUPPORTED
+    st = lib.segscan_spmm_backward_csr(byref(d), 0x9000, None, 8, 0xa000, 8, _abi.F32, 0xb000, 8, _abi.F32, None, 0,
+                                       0xc000, None, 0, None, 0, None)
+    assert st == _abi.EINVAL                                  # At.vals_map must be NULL
+    d.vals_map = None
+    st = lib.segscan_spmm_backward_csr(byref(d), 0x9000, None, 8, 0xa000, 8, _abi.F32, 0xb000, 8, _abi.F32, None, 0,
+                                       0xc000, None, 4, None, 0, None)
+    assert st == _abi.EINVAL                                  # unknown flag
+    assert lib.segscan_invert_permutation(None, -1, None, None) == _abi.EINVAL
+    assert lib.segscan_invert_permutation(None, 0, None, None) == _abi.OK
+    assert lib.segscan_invert_permutation(None, 5, None, None) == _abi.EINVAL
