@@ -87,9 +87,59 @@ def parse():
                                            "batched_graph", "batched_graph_kernels"],
                     default=None,
                     help="run one comparison leg alone (for per-kernel profiles) and print its timing")
+    ap.add_argument("--gather-ceiling", action="store_true",
+                    help="only measure the random-row read ceiling (segscan_probe_gather) and print it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB fp32 device-to-device 5 times (scales HBM byte counters)")
     return ap.parse_args()
+
+
+def gather_ceiling(nrows: int = 262144, n: int = 4194295, reps: int = 20) -> dict:
+    """Random dense-row read rate (segscan_probe_gather): n uniformly random rows of 256 B
+    and of 128 B from an nrows-row table, the SpMM's gather without its CSR streams and
+    output. Best over chunk lengths; HIP events around each probe kernel, median of reps."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from segscan_amd import _abi
+    lib = _abi.load()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(0)
+    idx = torch.from_numpy(rng.integers(0, nrows, size=n).astype(np.int32)).to(dev)
+    res = {}
+    for row_bytes in (256, 128):
+        table = torch.zeros(nrows * row_bytes // 4, dtype=torch.float32, device=dev)
+        best = None
+        for chunk in (16, 64, 256, 1024):
+            out_len = ctypes.c_int64()
+            _abi.check(lib.segscan_probe_gather_out_len(row_bytes, n, chunk, ctypes.byref(out_len)), "probe")
+            out = torch.empty(out_len.value, dtype=torch.float32, device=dev)
+
+            def call():
+                _abi.check(lib.segscan_probe_gather(ctypes.c_void_p(table.data_ptr()), nrows, row_bytes,
+                                                    ctypes.c_void_p(idx.data_ptr()), n, chunk,
+                                                    ctypes.c_void_p(out.data_ptr()), out_len.value,
+                                                    ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                           "probe_gather")
+            for _ in range(3):
+                call()
+            ts = []
+            for _ in range(reps):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                call()
+                b.record()
+                b.synchronize()
+                ts.append(a.elapsed_time(b))
+            ms = float(np.median(ts))
+            if best is None or ms < best[0]:
+                best = (ms, chunk)
+        gbps = n * row_bytes / (best[0] * 1e-3) / 1e9
+        res[f"random_{row_bytes}B"] = {"us": round(best[0] * 1e3, 2), "GBps": round(gbps, 1), "chunk": best[1]}
+    res["workload"] = f"{n} uniformly random rows of a {nrows}-row table"
+    return res
 
 
 def copy_calibration():
@@ -188,6 +238,9 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.gather_ceiling:
+        print(json.dumps({"gather_ceiling": gather_ceiling()}))
         return
     if args.only_leg:
         only_leg(args)
@@ -409,6 +462,16 @@ def main():
 
     achieved = b_alg(4) / (kern_ms * 1e-3) / 1e9
     counted = counted_traffic(info32["variant"], k32, N)
+    # the fabric's random 256-byte row read rate, against which the main kernel's measured
+    # traffic rate is set (its B gather is one random 256-byte row per nnz)
+    ceil = gather_ceiling(nrows=m.ncol, n=m.nnz)
+    log(f"gather ceiling: {ceil}")
+    ceiling = {"random_256B_rows_GBps": ceil["random_256B"]["GBps"], "random_128B_rows_GBps": ceil["random_128B"]["GBps"],
+               "source": "segscan_probe_gather, " + ceil["workload"]}
+    if counted is not None:
+        rate = counted["traffic_bytes"] / (kern_ms * 1e-3) / 1e9
+        ceiling["kernel_traffic_GBps"] = round(rate, 1)
+        ceiling["frac_of_ceiling"] = round(rate / ceil["random_256B"]["GBps"], 3)
     line = {
         "metric": "SpMM GB/s (% HBM peak) + fwd+bwd ms, N=64: tuned bf16 vs plain fp32",
         "value": round(value, 1),
@@ -434,6 +497,7 @@ def main():
                      "traffic_over_algorithmic": None if counted is None else
                      round(counted["traffic_bytes"] / b_alg(4), 2),
                      "traffic_source": None if counted is None else str(COUNTER_SUMMARY.relative_to(ROOT)),
+                     "ceiling": ceiling,
                      "kernel": KERNEL_OF_VARIANT[info32["variant"]],
                      "kernel_us": round(kern_ms * 1e3, 2), "algorithmic_bytes": b_alg(4)},
         "cpu_baseline": cpu,

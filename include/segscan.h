@@ -377,6 +377,17 @@ segscan_status_t segscan_check_csr(const segscan_csr_t* A, int32_t* flags_dev, v
 segscan_status_t segscan_check_index(const int32_t* index, int64_t E, int64_t dst_len, int32_t* flags_dev,
                                      void* stream);
 
+/* ---- gather probe (measurement) -----------------------------------------------------------
+ * Reads rows idx[i] (i < n) of a dense table (table_rows x row_bytes, row_bytes 128 or 256)
+ * as the SpMM's dense-row gather does: groups of row_bytes / 16 lanes, 16 bytes per lane,
+ * 16 rows in flight per group, `chunk` indices per group; writes one checksum per lane to
+ * out (segscan_probe_gather_out_len floats). The indices are not checked: each must be
+ * below table_rows. It gives the random-row read ceiling the SpMM kernels are compared
+ * with; no counterpart in the reference. The timing hook below times it too. */
+segscan_status_t segscan_probe_gather_out_len(int64_t row_bytes, int64_t n, int64_t chunk, int64_t* out_len);
+segscan_status_t segscan_probe_gather(const void* table, int64_t table_rows, int64_t row_bytes, const int32_t* idx,
+                                      int64_t n, int64_t chunk, float* out, int64_t out_len, void* stream);
+
 /* ---- measurement hook ---------------------------------------------------------------------
  * With non-NULL hipEvent_t handles, every later segscan_spmm_csr / segscan_segment_coo /
  * segscan_spmm_backward_csr call on this host thread records `start` on its stream right before its main kernel

@@ -137,6 +137,9 @@ _SIGNATURES = {
     "segscan_autotune_spmm": (c_int32, [c_void_p, POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p,
                                         c_int64, c_int32, c_void_p, c_size_t, c_void_p, POINTER(c_int32),
                                         POINTER(Knobs), POINTER(ctypes.c_float)]),
+    "segscan_probe_gather_out_len": (c_int32, [c_int64, c_int64, c_int64, POINTER(c_int64)]),
+    "segscan_probe_gather": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_int64, c_int64, c_void_p, c_int64,
+                                       c_void_p]),
     "segscan_invert_permutation": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
     "segscan_spmm_csr_noalloc": (c_int32, [POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p, c_int64,
                                            c_int32, POINTER(Knobs), c_uint32, c_void_p]),

@@ -210,6 +210,8 @@ hipError_t launch_segment_gather(const float* ddst, int64_t ld_ddst, const int32
 hipError_t launch_segment_counts(const int32_t* index, int64_t E, int64_t dst_len, int32_t* counts,
                                  hipStream_t s);
 hipError_t launch_invert_permutation(const int32_t* perm, int64_t n, int32_t* inv, hipStream_t s);
+hipError_t launch_gather_probe(const void* table, int64_t row_bytes, const int32_t* idx, int64_t n, int64_t chunk,
+                               float* out, int64_t out_len, hipStream_t s);
 hipError_t launch_check_csr(const int32_t* rowptr, const int32_t* colind, int64_t M, int64_t K, int64_t nnz,
                             int32_t* flags, hipStream_t s);
 hipError_t launch_check_index(const int32_t* index, int64_t E, int64_t dst_len, int32_t* flags, hipStream_t s);

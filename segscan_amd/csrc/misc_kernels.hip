@@ -540,6 +540,52 @@ hipError_t launch_segment_counts(const int32_t* index, int64_t E, int64_t dst_le
   return hipGetLastError();
 }
 
+// ---- gather probe: the fabric rate of random dense-row reads ----
+// Each group of G lanes reads rows idx[i] of the table (16 bytes per lane, 16 loads in
+// flight per group) over its chunk of indices and keeps a running sum, written once per
+// lane at the end so that no load is dropped. It reads as the SpMM's dense-row gather does,
+// without the CSR streams or the output.
+
+template <int G>
+__global__ void __launch_bounds__(256) gather_probe_kernel(const uint4* __restrict__ table, int64_t row_vecs,
+                                                           const int32_t* __restrict__ idx, int64_t n, int64_t chunk,
+                                                           float* __restrict__ out) {
+  constexpr int U = 16;
+  const int gl = threadIdx.x % G;
+  const int64_t group = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+  const int64_t i0 = group * chunk;
+  const int64_t i1 = min(n, i0 + chunk);
+  uint32_t acc = 0;
+  for (int64_t i = i0; i < i1; i += U) {
+    uint4 r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = (i + u < i1) ? idx[i + u] : idx[i];
+      r[u] = table[k * row_vecs + gl];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (i + u < i1) acc += r[u].x ^ r[u].y ^ r[u].z ^ r[u].w;
+    }
+  }
+  out[static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x] = static_cast<float>(acc & 0xffff);
+}
+
+hipError_t launch_gather_probe(const void* table, int64_t row_bytes, const int32_t* idx, int64_t n, int64_t chunk,
+                               float* out, int64_t out_len, hipStream_t s) {
+  const int G = static_cast<int>(row_bytes / 16);
+  const int64_t groups = (n + chunk - 1) / chunk;
+  const int64_t blocks = (groups * G + 255) / 256;
+  if (blocks * 256 > out_len) return hipErrorInvalidValue;
+  const uint4* t = static_cast<const uint4*>(table);
+  switch (G) {
+    case 8: gather_probe_kernel<8><<<static_cast<unsigned>(blocks), 256, 0, s>>>(t, G, idx, n, chunk, out); break;
+    case 16: gather_probe_kernel<16><<<static_cast<unsigned>(blocks), 256, 0, s>>>(t, G, idx, n, chunk, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // ---- inverse permutation: inv[perm[i]] = i ----
 
 __global__ void invert_permutation_kernel(const int32_t* __restrict__ perm, int64_t n, int32_t* __restrict__ inv) {

@@ -1252,6 +1252,30 @@ segscan_status_t segscan_segment_gather(const float* ddst, int64_t ld_ddst, cons
                     "segment gather");
 }
 
+segscan_status_t segscan_probe_gather_out_len(int64_t row_bytes, int64_t n, int64_t chunk, int64_t* out_len) {
+  if (!out_len || n < 0 || chunk < 1 || (row_bytes != 128 && row_bytes != 256))
+    return fail(SEGSCAN_EINVAL, "row_bytes must be 128 or 256, chunk >= 1, n >= 0");
+  const int64_t groups = (n + chunk - 1) / chunk;
+  *out_len = std::max<int64_t>(1, (groups * (row_bytes / 16) + 255) / 256) * 256;
+  return ok();
+}
+
+segscan_status_t segscan_probe_gather(const void* table, int64_t table_rows, int64_t row_bytes, const int32_t* idx,
+                                      int64_t n, int64_t chunk, float* out, int64_t out_len, void* stream) {
+  int64_t need = 0;
+  segscan_status_t st = segscan_probe_gather_out_len(row_bytes, n, chunk, &need);
+  if (st) return st;
+  if (n == 0) return ok();
+  if (!table || !idx || !out || table_rows < 1) return fail(SEGSCAN_EINVAL, "NULL pointer or empty table");
+  if (reinterpret_cast<uintptr_t>(table) % 16) return fail(SEGSCAN_EINVAL, "table is not 16-byte aligned");
+  if (out_len < need) return fail(SEGSCAN_EINVAL, "out holds %lld floats, %lld needed", (long long)out_len,
+                                  (long long)need);
+  mark_start(static_cast<hipStream_t>(stream));
+  hipError_t e = launch_gather_probe(table, row_bytes, idx, n, chunk, out, out_len, static_cast<hipStream_t>(stream));
+  mark_stop(static_cast<hipStream_t>(stream));
+  return hip_status(e, "gather probe");
+}
+
 segscan_status_t segscan_invert_permutation(const int32_t* perm, int64_t n, int32_t* inv, void* stream) {
   if (n < 0 || n > kMaxI32) return fail(SEGSCAN_EINVAL, "n out of range");
   if (n == 0) return ok();

@@ -331,3 +331,13 @@ def test_values_order_validation(lib):
     assert lib.segscan_invert_permutation(None, -1, None, None) == _abi.EINVAL
     assert lib.segscan_invert_permutation(None, 0, None, None) == _abi.OK
     assert lib.segscan_invert_permutation(None, 5, None, None) == _abi.EINVAL
+
+
+def test_gather_probe_validation(lib):
+    n = ctypes.c_int64()
+    assert lib.segscan_probe_gather_out_len(64, 100, 16, byref(n)) == _abi.EINVAL   # 128 or 256 bytes only
+    assert lib.segscan_probe_gather_out_len(256, 100, 0, byref(n)) == _abi.EINVAL
+    assert lib.segscan_probe_gather_out_len(256, 1000, 16, byref(n)) == _abi.OK
+    assert n.value == 1024  # 63 groups x 16 lanes, rounded up to whole workgroups of 256
+    assert lib.segscan_probe_gather(0x1000, 10, 256, 0x2000, 1000, 16, 0x3000, 100, None) == _abi.EINVAL  # out short
+    assert lib.segscan_probe_gather(0x1008, 10, 256, 0x2000, 1000, 16, 0x3000, 1024, None) == _abi.EINVAL  # misaligned
