@@ -18,7 +18,8 @@
 // before consuming any. The B row of the current At row stays in registers and
 // the next one is in flight (At rows are consecutive unless empty ones lie in
 // between, which are then loaded on demand). dB rows use the owner policy (a chunk
-// stores the rows it owns, zero-fills its empty ones, and sends a first row
+// stores the rows it owns, zeroes its empty ones but whole entry-free blocks of
+// them (left to the fix-up pass), and sends a first row
 // that began in an earlier chunk to carry_val for the fix-up pass). Each dvals
 // entry is written once (perm is a bijection) after a DPP reduction across the
 // group. Where At rows are short (a few entries each) the one-row-ahead B load
@@ -54,6 +55,8 @@ __device__ __forceinline__ void bwd_walk(const BwdParams& p, int32_t* bwd_smem) 
   const bool want_dv = p.dvals != nullptr;
   const bool acc_mode = p.accumulate != 0;
   const bool need_perm = (!p.vals_t && p.vals) || (want_dv && !p.dvals_at_order);
+  // empty dB rows: partial blocks zeroed here, entry-free blocks by the fix-up pass
+  const bool zrun = want_db && !acc_mode;
 
   for (int64_t c = ggid;; c += p.total_groups) {
     if (__ballot(c < p.nchunks) == 0) break;
@@ -78,8 +81,9 @@ __device__ __forceinline__ void bwd_walk(const BwdParams& p, int32_t* bwd_smem) 
     const bool is_carry = chunk_valid && c0 > 0 && prev_row == first_row;
     if (chunk_valid && want_db) {
       if (g_lane == 0) p.carry_row[c] = is_carry ? first_row : -1;
-      if (!acc_mode && nv > 0) {
-        for (int r = prev_row + 1; r < first_row; ++r) zero_cols<CF>(p.dB + static_cast<int64_t>(r) * p.ld_db + col0, nv);
+      if (zrun) {
+        // the run between the previous chunk's last row (-1 for chunk 0) and this one's first
+        owner_empty_run<CF, G, false>(p.dB, p.ld_db, col0, nv, p.K, prev_row, first_row, nullptr, false, g_lane);
       }
     }
     int cur = first_row;
@@ -174,8 +178,8 @@ __device__ __forceinline__ void bwd_walk(const BwdParams& p, int32_t* bwd_smem) 
             const int r = s_r[sbase + pp + u];
             if (r != cur) {
               flush(cur);
-              if (want_db && !acc_mode && nv > 0) {
-                for (int z = cur + 1; z < r; ++z) zero_cols<CF>(p.dB + static_cast<int64_t>(z) * p.ld_db + col0, nv);
+              if (zrun) {
+                owner_empty_run<CF, G, false>(p.dB, p.ld_db, col0, nv, p.K, cur, r, nullptr, false, g_lane);
               }
               if (want_dv) {
                 if constexpr (RB) {
@@ -223,10 +227,8 @@ __device__ __forceinline__ void bwd_walk(const BwdParams& p, int32_t* bwd_smem) 
     }
     if (chunk_valid && clen > 0) {
       flush(cur);
-      // The last chunk owns the trailing empty rows.
-      if (want_db && !acc_mode && nv > 0 && c == p.nchunks - 1) {
-        for (int z = cur + 1; z < p.K; ++z) zero_cols<CF>(p.dB + static_cast<int64_t>(z) * p.ld_db + col0, nv);
-      }
+      // the last chunk owns the run after the last row with entries
+      if (zrun && c == p.nchunks - 1) owner_empty_run<CF, G, false>(p.dB, p.ld_db, col0, nv, p.K, cur, p.K, nullptr, false, g_lane);
     }
   }
 }

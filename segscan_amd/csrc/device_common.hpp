@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "kernel_params.hpp"
+
 namespace segscan {
 
 constexpr int kWave = 64;
@@ -224,44 +226,6 @@ __device__ __forceinline__ uint64_t group_mask() {
   else return (1ull << G) - 1ull;
 }
 
-// Group-cooperative lookup for an aligned group of G lanes walking consecutive
-// nnz of one chunk. `rp` holds rowptr[r_hint + 1 + g_lane] (INT32_MAX past M),
-// loaded ahead of time by the caller so that its latency overlaps other work.
-// Precondition: rowptr[r_hint] <= e for every active lane, e < rowptr[M].
-// Each lane counts the boundaries <= e in the window with a log2(G)-step
-// binary search over the group's lanes; lanes past the window step to the next
-// G boundaries (group-uniform loop, rare when rows average more than a few nnz).
-template <int G>
-__device__ __forceinline__ int group_row_find(const int32_t* __restrict__ rowptr, int M, int r_hint, int rp, int e,
-                                              bool active) {
-  const int lane = __lane_id();
-  const int gbase = lane & ~(G - 1);
-  const int g_lane = lane - gbase;
-  int base = r_hint + 1;
-  int row = r_hint;
-  bool done = !active;
-  while (true) {
-    int pos = 0;
-#pragma unroll
-    for (int s = G / 2; s >= 1; s >>= 1) {
-      const int v = __shfl(rp, gbase + pos + s - 1);
-      if (v <= e) pos += s;
-    }
-    const int last = __shfl(rp, gbase + G - 1);
-    if (pos == G - 1 && last <= e) pos = G;
-    if (!done && pos < G) {
-      row = base + pos - 1;
-      done = true;
-    }
-    const uint64_t pending = __ballot(!done);
-    if (((pending >> gbase) & group_mask<G>()) == 0) break;
-    base += G;
-    const int j = base + g_lane;
-    rp = (j <= M) ? rowptr[j] : INT32_MAX;
-  }
-  return row;
-}
-
 // Wave-cooperative 64-ary search (every lane calls it with the same arguments,
 // full EXEC): the first j in [lo, hi] with a[j] > key, given a non-decreasing
 // and a[hi] > key. Each round probes 64 evenly spaced entries with one load per
@@ -306,6 +270,66 @@ __device__ __forceinline__ int64_t group_search_gt(const int32_t* __restrict__ a
   return hi;
 }
 
+// The first j in [lo, hi] with a[j] > key (a non-decreasing, a[hi] > key; lo / hi / key
+// group-uniform): a G-ary search by the group, a binary search for narrower groups.
+template <int G>
+__device__ __forceinline__ int first_gt(const int32_t* __restrict__ a, int lo, int hi, int key) {
+  if constexpr (G >= 8) {
+    return static_cast<int>(group_search_gt<G>(a, lo, hi, key));
+  } else {
+    while (lo < hi) {
+      const int mid = lo + ((hi - lo) >> 1);
+      if (a[mid] > key) hi = mid;
+      else lo = mid + 1;
+    }
+    return hi;
+  }
+}
+
+// Group-cooperative lookup for an aligned group of G lanes walking consecutive
+// nnz of one chunk. `rp` holds rowptr[r_hint + 1 + g_lane] (INT32_MAX past M),
+// loaded ahead of time by the caller so that its latency overlaps other work.
+// Precondition: rowptr[r_hint] <= e for every active lane, e < rowptr[M].
+// Each lane counts the boundaries <= e in the window with a log2(G)-step
+// binary search over the group's lanes; lanes past the window step to the next
+// G boundaries (group-uniform loop, rare when rows average more than a few nnz),
+// or, when those too lie at or below the lowest pending entry (a run of empty
+// rows), jump to that entry's row with a search instead of walking the run.
+template <int G>
+__device__ __forceinline__ int group_row_find(const int32_t* __restrict__ rowptr, int M, int r_hint, int rp, int e,
+                                              bool active) {
+  const int lane = __lane_id();
+  const int gbase = lane & ~(G - 1);
+  const int g_lane = lane - gbase;
+  int base = r_hint + 1;
+  int row = r_hint;
+  bool done = !active;
+  while (true) {
+    int pos = 0;
+#pragma unroll
+    for (int s = G / 2; s >= 1; s >>= 1) {
+      const int v = __shfl(rp, gbase + pos + s - 1);
+      if (v <= e) pos += s;
+    }
+    const int last = __shfl(rp, gbase + G - 1);
+    if (pos == G - 1 && last <= e) pos = G;
+    if (!done && pos < G) {
+      row = base + pos - 1;
+      done = true;
+    }
+    const uint64_t pending = (__ballot(!done) >> gbase) & group_mask<G>();
+    if (pending == 0) break;
+    // Pending lanes saw rowptr[base + G - 1] <= e < rowptr[M], so base + G <= M. The
+    // lowest pending lane holds the smallest pending entry (lanes walk consecutive ones).
+    base += G;
+    const int e_min = __shfl(e, gbase + __builtin_ctzll(pending));
+    if (rowptr[min(base + G - 1, M)] <= e_min) base = first_gt<G>(rowptr, base, M, e_min);
+    const int j = base + g_lane;
+    rp = (j <= M) ? rowptr[j] : INT32_MAX;
+  }
+  return row;
+}
+
 // Wave-cooperative lookup: every lane of the wave must call it (full EXEC).
 // Precondition: rowptr[r0] <= e for every lane and e < rowptr[M] (callers clamp
 // padding lanes to a valid position). The wave loads rowptr[r0+1 .. r0+64]
@@ -333,9 +357,58 @@ __device__ __forceinline__ int wave_row_lookup(const int32_t* __restrict__ rowpt
       done = true;
     }
     if (__ballot(!done) == 0) break;
+    // Pending lanes saw rowptr[base + 63] <= e < rowptr[M]. When the next 64 boundaries
+    // also lie at or below every pending entry (a run of empty rows), jump to the row
+    // of the smallest one with a 64-ary search instead of walking the run.
     base += 64;
+    int e_min = done ? INT32_MAX : e;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) e_min = min(e_min, __shfl_xor(e_min, s));
+    if (rowptr[min(base + 63, M)] <= e_min) base = static_cast<int>(wave_search_gt(rowptr, base, M, e_min));
   }
   return row;
+}
+
+// Empty rows under the owner write policy. Rows are grouped in blocks of kZeroRows. A
+// block whose rows are all empty is zeroed by the fix-up pass after the kernel, with its
+// whole grid; the kernel itself zeroes only the rows of an empty run that share a block
+// with a row with entries: at most 2 x (kZeroRows - 1) per run, whatever its length.
+// The fix-up pass finds entry-free blocks from rowptr (CSR: rowptr[16j] == rowptr[16j + 16])
+// or, for segments, from zflag[j], one byte per block, rewritten by every call: 0 when
+// block j holds a segment with entries (stored by a chunk that reaches it), 1 when it
+// holds none (stored by the owner of the empty run that covers it). Nothing needs
+// clearing between calls.
+//
+// The run of empty rows strictly between rows a and b (a = -1: before the first row;
+// b = M: after the last), seen by one group: lanes with nv > 0 zero their CF columns (at
+// col0) of the run's partial blocks; with FLAGS and `flags` the group's G lanes also flag
+// its whole blocks (only groups of the first column slice pass flags).
+template <int CF, int G, bool FLAGS>
+__device__ __forceinline__ void owner_empty_run(float* __restrict__ C, int64_t ldc, int col0, int nv, int M, int a,
+                                                int b, uint8_t* __restrict__ zflag, bool flags, int g_lane) {
+  const int s = a + 1;
+  if (s >= b) return;
+  const int j0 = (s + kZeroRows - 1) >> kZeroRowsLog;                            // first block starting at s or later
+  const int j1 = b >= M ? static_cast<int>(zero_blocks(M)) : (b >> kZeroRowsLog);  // blocks below j1 end by b
+  int z0 = b, z1 = b;  // rows [s, z0) and [z1, b) are zeroed here
+  if (j0 < j1) {
+    z0 = j0 << kZeroRowsLog;
+    z1 = min(b, j1 << kZeroRowsLog);
+    if constexpr (FLAGS) {
+      if (flags) {
+        // bytes up to a 4-aligned block, then 4 blocks per 32-bit store, then the rest
+        const int w0 = min(j1, (j0 + 3) & ~3), w1 = max(w0, j1 & ~3);
+        for (int j = j0 + g_lane; j < w0; j += G) zflag[j] = 1;
+        uint32_t* zw = reinterpret_cast<uint32_t*>(zflag);
+        for (int w = (w0 >> 2) + g_lane; w < (w1 >> 2); w += G) zw[w] = 0x01010101u;
+        for (int j = w1 + g_lane; j < j1; j += G) zflag[j] = 1;
+      }
+    }
+  }
+  if (nv > 0) {
+    for (int r = s; r < z0; ++r) zero_cols<CF>(C + static_cast<int64_t>(r) * ldc + col0, nv);
+    for (int r = z1; r < b; ++r) zero_cols<CF>(C + static_cast<int64_t>(r) * ldc + col0, nv);
+  }
 }
 
 }  // namespace segscan

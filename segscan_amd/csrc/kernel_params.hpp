@@ -12,6 +12,12 @@ namespace segscan {
 
 enum class VType : int { F32 = 0, BF16 = 1, F16 = 2 };
 
+// Owner write policy: empty output rows are tracked in blocks of kZeroRows rows, one
+// flag byte per block (owner_empty_run, carry_fixup_kernel).
+constexpr int kZeroRowsLog = 4;
+constexpr int kZeroRows = 1 << kZeroRowsLog;
+__host__ __device__ constexpr int64_t zero_blocks(int64_t rows) { return (rows + kZeroRows - 1) >> kZeroRowsLog; }
+
 // Parallel-reduction SpMM / segment_coo (ebpr, rbpr, segment_coo, segment_coo_new).
 // Lanes of a workgroup are laid out [Y][P][X]: X lanes over consecutive nnz
 // (the tile), P column panels of CF columns, Y tiles per workgroup.
@@ -61,6 +67,7 @@ struct SrParams {
   const int32_t* chunk_prev;  // row of the nnz just before each chunk, -1 for chunk 0 (SpMM)
   int32_t* carry_row;         // [nchunks] owner policy: row of the chunk's carry or -1
   float* carry_val;           // [nchunks x N] owner policy
+  uint8_t* zflag;             // [zero_blocks(M)] segments, owner policy, not accumulating: entry-free blocks
   int policy;                 // 1 atomic, 2 owner store + carry, 3 lessatom
   int accumulate;
 };
@@ -156,8 +163,13 @@ hipError_t launch_spmm_bwd(VType xt, VType yt, int cf, int g, bool row_batch, co
 hipError_t launch_fill_zero(float* C, int64_t rows, int64_t N, int64_t ldc, hipStream_t s);
 hipError_t launch_partition_rows(const int32_t* rowptr, int M, int64_t nnz, int64_t chunk, int64_t nchunks,
                                  int32_t* chunk_row, int32_t* chunk_prev, hipStream_t s);
+// Fix-up pass of the owner write policy: adds each chunk's carry to its row and zeroes
+// every row of the entry-free blocks of kZeroRows rows: those flagged in
+// zflag[0, zero_blocks(M)) when zflag is given, else those with rowptr[16j] ==
+// rowptr[min(16j + 16, M)] when rowptr is given (neither: no zeroing, as when accumulating).
 hipError_t launch_carry_fixup(const int32_t* carry_row, const float* carry_val, int64_t nchunks, int N,
-                              float* C, int64_t ldc, hipStream_t s);
+                              float* C, int64_t ldc, const uint8_t* zflag, const int32_t* rowptr, int64_t M,
+                              hipStream_t s);
 // CSR -> CSC transpose as a stable LSD radix sort of the column indices, carrying
 // (row << 32 | position) per entry. Tiles of kTransposeTile entries; `passes`
 // digits of `dbits` bits cover the `bits` bits of the largest column index.

@@ -76,11 +76,18 @@ hipError_t launch_partition_rows(const int32_t* rowptr, int M, int64_t nnz, int6
 
 // ---- carry fix-up of the owner write policy ----
 
+// Carries: one lane per (chunk, column). Entry-free blocks: each wave tests 64 blocks at
+// once and zeroes the rows of those that hold no entry (rows x N flattened over its
+// lanes); consecutive windows of 64 blocks go to consecutive waves, so one long empty
+// run is zeroed by many waves.
 __global__ void carry_fixup_kernel(const int32_t* __restrict__ carry_row, const float* __restrict__ carry_val,
-                                   int64_t nchunks, int N, float* __restrict__ C, int64_t ldc) {
+                                   int64_t nchunks, int N, float* __restrict__ C, int64_t ldc,
+                                   const uint8_t* __restrict__ zflag, const int32_t* __restrict__ rowptr,
+                                   int64_t M) {
   const int64_t n = nchunks * N;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+  const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (int64_t i = tid; i < n; i += stride) {
     const int64_t c = i / N;
     const int32_t r = carry_row[c];
     if (r >= 0) {
@@ -88,12 +95,49 @@ __global__ void carry_fixup_kernel(const int32_t* __restrict__ carry_row, const 
       unsafeAtomicAdd(C + static_cast<int64_t>(r) * ldc + j, carry_val[i]);
     }
   }
+  if (zflag == nullptr && rowptr == nullptr) return;
+  const int64_t nblk = zero_blocks(M);
+  const int lane = threadIdx.x & 63;
+  // Row stores: 16-byte units when rows allow them. A row of U units takes U lanes;
+  // 64 / U rows go at once when U <= 64, else the lanes loop over the row's units.
+  const bool vec = N % 4 == 0 && ldc % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0;
+  const int U = vec ? N / 4 : N;
+  const int rows_at_once = U <= 64 ? 64 / U : 1;
+  const int lane_row = U <= 64 ? lane / U : 0;
+  const int lane_unit = U <= 64 ? lane - lane_row * U : lane;
+  for (int64_t w = tid >> 6; w * 64 < nblk; w += stride >> 6) {
+    const int64_t j = w * 64 + lane;
+    bool empty = false;
+    if (j < nblk) {
+      if (zflag) empty = zflag[j] != 0;
+      else empty = rowptr[j << kZeroRowsLog] == rowptr[min((j + 1) << kZeroRowsLog, M)];
+    }
+    uint64_t m = __ballot(empty);
+    while (m) {
+      const int64_t blk = w * 64 + (__ffsll(static_cast<unsigned long long>(m)) - 1);
+      m &= m - 1;
+      const int64_t r0 = blk << kZeroRowsLog;
+      const int nrows = static_cast<int>(min(r0 + kZeroRows, M) - r0);
+      if (lane_row < rows_at_once) {
+        for (int rr = lane_row; rr < nrows; rr += rows_at_once) {
+          float* row = C + (r0 + rr) * ldc;
+          for (int u = lane_unit; u < U; u += (U <= 64 ? U : 64)) {
+            if (vec) *reinterpret_cast<float4*>(row + 4 * static_cast<int64_t>(u)) = make_float4(0.f, 0.f, 0.f, 0.f);
+            else row[u] = 0.0f;
+          }
+        }
+      }
+    }
+  }
 }
 
 hipError_t launch_carry_fixup(const int32_t* carry_row, const float* carry_val, int64_t nchunks, int N, float* C,
-                              int64_t ldc, hipStream_t s) {
+                              int64_t ldc, const uint8_t* zflag, const int32_t* rowptr, int64_t M, hipStream_t s) {
   if (nchunks <= 0 || N <= 0) return hipSuccess;
-  carry_fixup_kernel<<<grid_for(nchunks * N, kBlock), kBlock, 0, s>>>(carry_row, carry_val, nchunks, N, C, ldc);
+  // at least one wave per 64 blocks of flags, up to the carry work's grid or more
+  const int64_t waves = (zflag || rowptr) ? (zero_blocks(M) + 63) / 64 : 0;
+  const int64_t work = std::max<int64_t>(nchunks * N, waves * 64);
+  carry_fixup_kernel<<<grid_for(work, kBlock), kBlock, 0, s>>>(carry_row, carry_val, nchunks, N, C, ldc, zflag, rowptr, M);
   return hipGetLastError();
 }
 

@@ -425,10 +425,10 @@ struct SegPlan {
   int X = 1, P = 1, Y = 1, slice = 1, block = 64, tpb = 64;
   int64_t grid_x = 1, grid_y = 1, ntiles = 0, chunk = 0, nchunks = 0, total_groups = 0;
   bool zero_fill = true;
-  size_t ws_bytes = 0, off_crow = 0, off_cval = 0;
+  size_t ws_bytes = 0, off_crow = 0, off_cval = 0, off_zflag = 0;
 };
 
-segscan_status_t plan_seg(int64_t E, int64_t N, segscan_dtype_t st, int variant, const Knobs& user, int cf_cap,
+segscan_status_t plan_seg(int64_t E, int64_t N, int64_t dst_len, segscan_dtype_t st, int variant, const Knobs& user, int cf_cap,
                           SegPlan& pl) {
   if (variant < 0 || variant > SEGSCAN_SEG_AUTO) return fail(SEGSCAN_EINVAL, "unknown segment_coo variant %d", variant);
   const int vec = vec_elems(st);
@@ -604,6 +604,8 @@ segscan_status_t plan_seg(int64_t E, int64_t N, segscan_dtype_t st, int variant,
       off += align_up(static_cast<size_t>(std::max<int64_t>(pl.nchunks, 1)) * 4);
       pl.off_cval = off;
       off += align_up(static_cast<size_t>(std::max<int64_t>(pl.nchunks, 1)) * static_cast<size_t>(N) * 4);
+      pl.off_zflag = off;  // one flag per block of kZeroRows segments
+      off += align_up(static_cast<size_t>(zero_blocks(dst_len)));
     }
     pl.ws_bytes = off;
   }
@@ -812,7 +814,8 @@ segscan_status_t spmm_csr_impl(const segscan_csr_t* A, int64_t N, const void* B,
       e = launch_spmm_sr(vt, pl.CF, pl.G, p, l, compiled);
       mark_stop(s);
       if (e == hipSuccess && compiled && pl.policy == SEGSCAN_WRITE_OWNER)
-        e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, C, ldc, s);
+        e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, C, ldc, nullptr,
+                               accumulate ? nullptr : A->rowptr, M, s);
       break;
     }
     case kRbsr: {
@@ -857,7 +860,7 @@ segscan_status_t segscan_segment_coo_describe(int64_t E, int64_t N, int64_t dst_
   if (dst_len < 0 || dst_len >= kMaxI32) return fail(SEGSCAN_EINVAL, "dst_len = %lld out of range", (long long)dst_len);
   if (!dtype_valid(src_dtype)) return fail(SEGSCAN_EINVAL, "unknown dtype %d", (int)src_dtype);
   SegPlan pl;
-  segscan_status_t st = plan_seg(E, std::max<int64_t>(N, 1), src_dtype, variant, from_c(knobs), 8, pl);
+  segscan_status_t st = plan_seg(E, std::max<int64_t>(N, 1), dst_len, src_dtype, variant, from_c(knobs), 8, pl);
   if (st) return st;
   fill_info(pl, info);
   return ok();
@@ -899,7 +902,7 @@ segscan_status_t segscan_segment_coo(const void* src, int64_t ld_src, segscan_dt
   const bool accumulate = flags & SEGSCAN_ACCUMULATE;
   const int cap = aligned_cf(8, src, std::max<int64_t>(ld_src, 1), dtype_size(src_dtype), dst, std::max<int64_t>(ld_dst, 1));
   SegPlan pl;
-  segscan_status_t st = plan_seg(E, std::max<int64_t>(N, 1), src_dtype, variant, from_c(knobs), cap, pl);
+  segscan_status_t st = plan_seg(E, std::max<int64_t>(N, 1), dst_len, src_dtype, variant, from_c(knobs), cap, pl);
   if (st) return st;
   if (dst_len == 0 || N == 0) return ok();
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -964,13 +967,14 @@ segscan_status_t segscan_segment_coo(const void* src, int64_t ld_src, segscan_dt
     if (pl.policy == SEGSCAN_WRITE_OWNER) {
       p.carry_row = reinterpret_cast<int32_t*>(ws + pl.off_crow);
       p.carry_val = reinterpret_cast<float*>(ws + pl.off_cval);
+      p.zflag = accumulate ? nullptr : reinterpret_cast<uint8_t*>(ws + pl.off_zflag);
     }
     mark_start(s);
     l.shmem = static_cast<size_t>(pl.tpb) * 12;
     e = launch_seg_sr(vt, pl.CF, pl.G, p, l, compiled);
     mark_stop(s);
     if (e == hipSuccess && compiled && pl.policy == SEGSCAN_WRITE_OWNER)
-      e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, dst, ld_dst, s);
+      e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, dst, ld_dst, p.zflag, nullptr, dst_len, s);
   }
   if (!compiled)
     return fail(SEGSCAN_EUNSUPPORTED, "no compiled kernel for dtype %d, coarsen %d, group %d", (int)src_dtype, pl.CF, pl.G);
@@ -1217,7 +1221,9 @@ static segscan_status_t spmm_backward_impl(const segscan_csr_t* At, const int32_
   mark_stop(s);
   if (!compiled)
     return fail(SEGSCAN_EUNSUPPORTED, "no compiled backward kernel for coarsen %d, group %d", pl.CF, pl.G);
-  if (e == hipSuccess && dB) e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, dB, ld_db, s);
+  if (e == hipSuccess && dB)
+    e = launch_carry_fixup(p.carry_row, p.carry_val, pl.nchunks, p.N, dB, ld_db, nullptr, accumulate ? nullptr : At->rowptr,
+                           K, s);
   return hip_status(e, "fused backward");
 }
 

@@ -16,10 +16,11 @@
 //
 // Write policies:
 //  * atomic: every flushed row is an atomic add (output zero-filled beforehand);
-//  * owner:  chunk c owns rows (row(c0-1), row(c1-1)] — including empty rows,
-//            which it zero-fills — and stores each once; a row that started in an
-//            earlier chunk is a carry, written to carry_val[c] and added by a
-//            fix-up pass. No atomics on the main path and no zero-fill pass;
+//  * owner:  chunk c owns rows (row(c0-1), row(c1-1)] — including empty rows
+//            (zeroed as owner_empty_run describes) — and stores each once; a row
+//            that started in an earlier chunk is a carry, written to carry_val[c]
+//            and added by a fix-up pass. No atomics on the main path and no
+//            zero-fill pass over the output;
 //  * lessatom: rows that start and end inside the chunk are plain stores, the
 //            chunk's first and last rows atomic adds (output zero-filled first).
 #pragma once
@@ -63,6 +64,10 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   const int sbase = g * G;  // this group's LDS slots
   const bool acc_mode = p.accumulate != 0;
   const bool owner = p.policy == 2;
+  // empty runs: rows zeroed here, whole blocks flagged for the fix-up pass
+  // (segments flag entry-free blocks; for a CSR matrix the fix-up pass reads rowptr)
+  const bool zrun = owner && !acc_mode;
+  const bool zflags = SEG && zrun && blockIdx.y == 0;
 
   for (int64_t c = ggid;; c += p.total_groups) {
     // Chunks grow with the lane index; the wave's first lane has the lowest one.
@@ -96,8 +101,10 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
     const bool is_carry = owner && chunk_valid && c0 > 0 && prev_row == first_row;
     if (owner && chunk_valid) {
       if (g_lane == 0 && blockIdx.y == 0) p.carry_row[c] = is_carry ? first_row : -1;
-      if (!acc_mode && nv > 0) {
-        for (int r = prev_row + 1; r < first_row; ++r) zero_cols<CF>(p.C + static_cast<int64_t>(r) * p.ldc + col0, nv);
+      if (zrun) {
+        // the run between the previous chunk's last row (-1 for chunk 0) and this one's first
+        if (SEG && zflags && g_lane == 0) p.zflag[first_row >> kZeroRowsLog] = 0;
+        owner_empty_run<CF, G, SEG>(p.C, p.ldc, col0, nv, p.M, prev_row, first_row, p.zflag, zflags, g_lane);
       }
     }
     int cur = first_row;
@@ -145,15 +152,36 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
     auto consume = [&](int r, float v, const float (&b)[CF]) {
       if (r != cur) {
         flush(cur, false);
-        if (owner && !acc_mode && nv > 0) {
-          for (int z = cur + 1; z < r; ++z) zero_cols<CF>(p.C + static_cast<int64_t>(z) * p.ldc + col0, nv);
-        }
+        // rows of the run in partial blocks (segments: stage() flags the whole ones)
+        if (zrun) owner_empty_run<CF, G, false>(p.C, p.ldc, col0, nv, p.M, cur, r, nullptr, false, g_lane);
         cur = r;
 #pragma unroll
         for (int i = 0; i < CF; ++i) acc[i] = 0.0f;
       }
 #pragma unroll
       for (int i = 0; i < CF; ++i) acc[i] += v * b[i];
+    };
+
+    // Segments under the owner policy: block flags of the tile's entries, lane-parallel
+    // (each lane compares its segment with the previous entry's). A lane whose segment
+    // starts a new block stores the block's 0; a run between two entries that covers
+    // whole blocks is flagged by the group. The run before the chunk's first entry is
+    // the chunk start's.
+    int tprev = first_row;
+    auto flag_tile = [&](int row, bool valid, int cnt) {
+      const int up = __shfl(row, __lane_id() - (g_lane > 0 ? 1 : 0));
+      const int rp = g_lane > 0 ? up : tprev;
+      const bool fresh = valid && row != rp;
+      if (fresh && (row >> kZeroRowsLog) != (rp >> kZeroRowsLog)) p.zflag[row >> kZeroRowsLog] = 0;
+      const bool whole = fresh && ((rp + kZeroRows) >> kZeroRowsLog) < (row >> kZeroRowsLog);
+      uint64_t m = (__ballot(whole) >> wave_gbase) & (G == 64 ? ~0ull : ((1ull << G) - 1));
+      while (m) {
+        const int src = wave_gbase + __ffsll(static_cast<unsigned long long>(m)) - 1;
+        m &= m - 1;
+        const int ra = __shfl(rp, src), rb = __shfl(row, src);
+        owner_empty_run<1, G, true>(nullptr, 0, 0, 0, p.M, ra, rb, p.zflag, true, g_lane);
+      }
+      if (cnt > 0) tprev = __shfl(row, wave_gbase + cnt - 1);
     };
 
     // Stages the tile at `off`: rows found, (column, value, row) written to the
@@ -164,6 +192,7 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
       int row;
       if constexpr (SEG) {
         row = prow;
+        if (zflags) flag_tile(row, valid, cnt);
       } else {
         row = group_row_find<G>(p.rowptr, p.M, r_hint, prp, static_cast<int>(c0 + off + g_lane), valid);
         const int last_row = __shfl(row, wave_gbase + max(cnt, 1) - 1);
@@ -294,10 +323,8 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
     }
     if (chunk_valid && clen > 0) {
       flush(cur, true);
-      // The last chunk owns the trailing empty rows.
-      if (owner && !acc_mode && nv > 0 && c == p.nchunks - 1) {
-        for (int z = cur + 1; z < p.M; ++z) zero_cols<CF>(p.C + static_cast<int64_t>(z) * p.ldc + col0, nv);
-      }
+      // the last chunk owns the run after the last row with entries
+      if (zrun && c == p.nchunks - 1) owner_empty_run<CF, G, SEG>(p.C, p.ldc, col0, nv, p.M, cur, p.M, p.zflag, zflags, g_lane);
     }
   }
 }

@@ -87,5 +87,9 @@ def test_noalloc_matches_workspace_entry_point(gpu, oracle, case):
         C_ws = ops.spmm(Af, Bd, variant=variant, knobs=knobs)
         torch.cuda.synchronize()
         C64, D = oracle.spmm_f64(m.rowptr, m.colind, Af.vals.cpu().numpy(), Bf)
-        assert oracle.normalized_error(C.cpu().numpy(), C64, D) <= 1e-6
-        np.testing.assert_allclose(C.cpu().numpy(), C_ws.cpu().numpy(), rtol=1e-5, atol=1e-5)
+        got, got_ws = C.cpu().numpy(), C_ws.cpu().numpy()
+        assert oracle.normalized_error(got, C64, D) <= 1e-6
+        assert oracle.normalized_error(got_ws, C64, D) <= 1e-6
+        # atomic variants add in no fixed order: the two results agree within the sum of
+        # their bounds, 2e-6 x sum |a||b| per element (a fixed tolerance fails on long rows)
+        assert np.all(np.abs(got.astype(np.float64) - got_ws) <= 2e-6 * D), (variant, knobs, call)
