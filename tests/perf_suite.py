@@ -11,7 +11,9 @@ GPU as `python tests/perf_suite.py out.jsonl [legs]` from the repository root.
   gcn          D^-1/2 (A + I) D^-1/2, 131072 nodes, degree 20, X 131072 x 128 fp32 / bf16:
                forward, backward (A^T dC, SDDMM), mean-pool segment backward
   batched      256 graphs (64..50000 nodes, Zipf 2.1 degrees clipped at 2000), N = 64 bf16:
-               hipGraph replay of forward + pooling + backward vs the per-graph launch loop
+               hipGraph replay of forward + pooling + backward (the batch plan's one launch, and
+               kernel by kernel) vs the per-graph launch loop (plain, AUTO, table-tuned, and the
+               table-tuned loop captured as a hipGraph)
 
 Every timed leg uses HIP events on torch's current stream (median of `reps`
 after warm-up) with inputs resident in HBM. Errors are the normalized error
@@ -247,39 +249,57 @@ def run(out=None, legs=("poisson", "powerlaw", "segment", "banded", "gcn", "shar
         dP = torch.from_numpy(rng.standard_normal((b.n_graphs, N)).astype(np.float32)).to(dev)
         stats = autotune.row_stats(csr.rowptr)
         v, k, _ = _best(b.A, X, stats, reps=3, limit=60)
+        # the batch plan: forward, pooling and backward in one launch
         step = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, variant=v, knobs=k)
         eager_ms = _time(lambda: step.run(X, dP), reps=10)
-        g = step.capture(X, dP)
-        graph_ms = _time(g.replay, reps=20)
+        graph_ms = _time(step.capture(X, dP).replay, reps=20)
+        # kernel by kernel: SpMM, segment_coo pooling, fused backward reading dP through the graph index
+        kbk = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, variant=v, knobs=k, single_launch=False)
+        kbk_graph_ms = _time(kbk.capture(X, dP).replay, reps=20)
         # the fused backward's shape tuned for this batch (the built-in one is a candidate), on
-        # dP read through the graph index exactly as the step runs it
-        tb = autotune.tune_backward(step.T, None, dP, X, autotune.backward_candidates(N), reps=5,
-                                    vals_t=step.At.vals, dc_map=b.graph_index)
-        step_t = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, variant=v, knobs=k, knobs_bwd=tb[0]["knobs"])
-        graph_tuned_ms = _time(step_t.capture(X, dP).replay, reps=20)
-        sep = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, variant=v, knobs=k, fused=False)
-        g_sep = sep.capture(X, dP)
-        graph_sep_ms = _time(g_sep.replay, reps=20)  # transposed SpMM + SDDMM as two kernels
-        # forward-only comparison against the per-graph plain launch loop
-        fwd_ms = _time(lambda: (ops.spmm(b.A, X, v, k, out=step.H, workspace=step.ws_f),
-                                ops.segment_coo(step.H, b.graph_index, b.n_graphs, out=step.P, workspace=step.ws_s)),
-                       reps=10)
-        loop = batch.PerGraphLoop(csr, offsets, N, device=dev, variant="plain_ebpr")
-        Xf = X
+        # dP read through the graph index exactly as the kernel-by-kernel step runs it
+        tb = autotune.tune_backward(kbk.T, None, dP, X, autotune.backward_candidates(N), reps=5,
+                                    vals_t=kbk.At.vals, dc_map=b.graph_index)
+        kbk_t = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, variant=v, knobs=k, knobs_bwd=tb[0]["knobs"],
+                                     single_launch=False)
+        kbk_tuned_ms = _time(kbk_t.capture(X, dP).replay, reps=20)
+        sep = batch.BatchedGcnStep(b, N, dtype=torch.bfloat16, variant=v, knobs=k, fused=False, single_launch=False)
+        graph_sep_ms = _time(sep.capture(X, dP).replay, reps=20)  # transposed SpMM + SDDMM as two kernels
+        # forward + pooling: the plan with dP = None against the per-graph launch loops
+        fwd_ms = _time(lambda: step.plan.run(X, None, step.P, H=step.H), reps=10)
+        fwd_kbk_ms = _time(lambda: (ops.spmm(b.A, X, v, k, out=kbk.H, workspace=kbk.ws_f),
+                                    ops.segment_coo(kbk.H, b.graph_index, b.n_graphs, out=kbk.P, workspace=kbk.ws_s)),
+                           reps=10)
         H = torch.empty((csr.nrow, N), device=dev)
         P = torch.empty((b.n_graphs, N), device=dev)
-        loop_ms = _time(lambda: loop.run(Xf, H, P), reps=5, warmup=1)
-        loop_auto = batch.PerGraphLoop(csr, offsets, N, device=dev, variant="auto")
-        loop_auto_ms = _time(lambda: loop_auto.run(Xf, H, P), reps=5, warmup=1)
+        loop = batch.PerGraphLoop(csr, offsets, N, device=dev, variant="plain_ebpr", dtype=torch.bfloat16)
+        loop_ms = _time(lambda: loop.run(X, H, P), reps=5, warmup=1)
+        loop_auto = batch.PerGraphLoop(csr, offsets, N, device=dev, variant="auto", dtype=torch.bfloat16)
+        loop_auto_ms = _time(lambda: loop_auto.run(X, H, P), reps=5, warmup=1)
+        # each graph's variant and knobs from the tuning table's bucket (AUTO where none), run
+        # eagerly and captured as one hipGraph (no host issue cost left)
+        loop_tab = batch.PerGraphLoop(csr, offsets, N, device=dev, tuner=batch.table_tuner(torch.bfloat16, N),
+                                      dtype=torch.bfloat16)
+        loop_tab_ms = _time(lambda: loop_tab.run(X, H, P), reps=5, warmup=1)
+        torch.cuda.synchronize()
+        g_loop = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g_loop):
+            loop_tab.run(X, H, P)
+        loop_tab_graph_ms = _time(g_loop.replay, reps=10)
         _emit(out, {"leg": "batched_256_graphs_N64_bf16", "nodes": csr.nrow, "nnz": csr.nnz,
-                    "tuned": {"variant": v, "knobs": k}, "fwd_bwd_eager_ms": round(eager_ms, 4),
-                    "fwd_bwd_hipgraph_ms": round(graph_ms, 4), "fwd_bwd_hipgraph_separate_ms": round(graph_sep_ms, 4),
-                    "fwd_bwd_hipgraph_tuned_bwd_ms": round(graph_tuned_ms, 4), "bwd_knobs_tuned": tb[0]["knobs"],
+                    "tuned": {"variant": v, "knobs": k}, "plan": step.plan.describe(),
+                    "fwd_bwd_plan_eager_ms": round(eager_ms, 4), "fwd_bwd_plan_hipgraph_ms": round(graph_ms, 4),
+                    "fwd_bwd_kernel_by_kernel_hipgraph_ms": round(kbk_graph_ms, 4),
+                    "fwd_bwd_kernel_by_kernel_tuned_bwd_ms": round(kbk_tuned_ms, 4),
+                    "fwd_bwd_kernel_by_kernel_separate_bwd_ms": round(graph_sep_ms, 4),
+                    "bwd_knobs_tuned": tb[0]["knobs"],
                     "bwd_tuning": [{"knobs": r["knobs"], "ms": round(r["ms"], 4)} for r in tb],
-                    "fwd_pool_batched_ms": round(fwd_ms, 4),
+                    "fwd_pool_plan_ms": round(fwd_ms, 4), "fwd_pool_kernel_by_kernel_ms": round(fwd_kbk_ms, 4),
                     "fwd_pool_per_graph_loop_plain_ms": round(loop_ms, 4),
                     "fwd_pool_per_graph_loop_auto_ms": round(loop_auto_ms, 4),
-                    "speedup_batched_vs_loop": round(loop_ms / fwd_ms, 2)})
+                    "fwd_pool_per_graph_loop_table_ms": round(loop_tab_ms, 4),
+                    "fwd_pool_per_graph_loop_table_hipgraph_ms": round(loop_tab_graph_ms, 4),
+                    "speedup_plan_vs_table_loop_hipgraph": round(loop_tab_graph_ms / fwd_ms, 2)})
 
 
 if __name__ == "__main__":
