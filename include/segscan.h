@@ -40,12 +40,16 @@
 extern "C" {
 #endif
 
+/* Every enum ends in a *_MAX_ENUM = 0x7fffffff entry: its values then span all of int32
+ * (the type stays int-sized), so a caller's out-of-range value is a defined value that the
+ * library rejects with SEGSCAN_EINVAL, not undefined behaviour in the C++ implementation. */
 typedef enum segscan_status {
   SEGSCAN_OK = 0,
   SEGSCAN_EINVAL = 1,        /* null pointer, negative size, bad stride, misaligned operand */
   SEGSCAN_EUNSUPPORTED = 2,  /* knob combination outside the compiled set */
   SEGSCAN_EHIP = 3,          /* the HIP runtime reported an error (no device, launch failure) */
-  SEGSCAN_EWORKSPACE = 4     /* workspace missing or smaller than *_workspace_size reported */
+  SEGSCAN_EWORKSPACE = 4,    /* workspace missing or smaller than *_workspace_size reported */
+  SEGSCAN_STATUS_MAX_ENUM = 0x7fffffff
 } segscan_status_t;
 
 /* Storage type of the dense operand (B for SpMM, src for segment_coo).
@@ -53,7 +57,8 @@ typedef enum segscan_status {
 typedef enum segscan_dtype {
   SEGSCAN_F32 = 0,
   SEGSCAN_BF16 = 1,
-  SEGSCAN_F16 = 2
+  SEGSCAN_F16 = 2,
+  SEGSCAN_DTYPE_MAX_ENUM = 0x7fffffff
 } segscan_dtype_t;
 
 /* SpMM variants. EB = element (nnz) balanced, RB = row balanced,
@@ -72,7 +77,8 @@ typedef enum segscan_spmm_variant {
   SEGSCAN_SPMM_PLAIN_RBPR = 5,  /* csrspmm_parreduce_rowbalance */
   SEGSCAN_SPMM_PLAIN_EBSR = 6,  /* csrspmm_rowcaching_nnzbalance */
   SEGSCAN_SPMM_PLAIN_RBSR = 7,  /* csrspmm_seqreduce_rowbalance */
-  SEGSCAN_SPMM_AUTO = 8
+  SEGSCAN_SPMM_AUTO = 8,
+  SEGSCAN_SPMM_MAX_ENUM = 0x7fffffff
 } segscan_spmm_variant_t;
 
 /* segment_coo variants (new-cg/include/segscan/segscan.cuh). AUTO runs the
@@ -88,7 +94,8 @@ typedef enum segscan_seg_variant {
   SEGSCAN_SEG_NOSHMEM_SR = 3,       /* segment_coo_noshmem_sr */
   SEGSCAN_SEG_LESSATOM_SR = 4,      /* segment_coo_noshmem_lessatom_sr */
   SEGSCAN_SEG_SORTED_SR = 5,        /* segment_coo_sorted_sr */
-  SEGSCAN_SEG_AUTO = 6
+  SEGSCAN_SEG_AUTO = 6,
+  SEGSCAN_SEG_MAX_ENUM = 0x7fffffff
 } segscan_seg_variant_t;
 
 /* Write policy of the sequential-reduction kernels. */
@@ -98,9 +105,10 @@ typedef enum segscan_write_policy {
   SEGSCAN_WRITE_OWNER = 2,    /* rows are stored once by the chunk that owns them (empty rows
                                  zero-filled by their owner); a row that crosses a chunk
                                  boundary gets a carry added by a fix-up pass; needs workspace */
-  SEGSCAN_WRITE_LESSATOM = 3  /* rows that start and end inside a chunk are plain stores, the
+  SEGSCAN_WRITE_LESSATOM = 3, /* rows that start and end inside a chunk are plain stores, the
                                  chunk's first and last row are atomic adds (output zero-filled
                                  first); the reference's segment_coo_noshmem_lessatom_sr policy */
+  SEGSCAN_WRITE_MAX_ENUM = 0x7fffffff
 } segscan_write_policy_t;
 
 /* Tuning knobs, named as in the reference templates. A field left 0 takes the

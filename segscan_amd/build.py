@@ -83,6 +83,67 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     return LIB
 
 
+# Host-sanitized build: the translation units that hold the C ABI's host logic (argument
+# validation, planners, tuning tables, Matrix Market reader) are recompiled with
+# AddressSanitizer and UndefinedBehaviorSanitizer on their host pass only (device code is
+# unchanged); the kernel-only units are reused from _build/. Load it with the ASan runtime
+# preloaded: see sanitized_env().
+SAN_BUILD = PKG / "_build_san"
+SAN_LIB = SAN_BUILD / "libsegscan_hip.so"
+SAN_SOURCES = ["segscan_abi.cpp", "mtx_loader.cpp", "tune_table.hip", "noalloc.hip", "batch_kernels.hip"]
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=undefined", "-g"]
+
+
+def _compile_san(src: str) -> Path:
+    obj = SAN_BUILD / (Path(src).stem + ".o")
+    cmd = [hipcc(), *CXXFLAGS, *SAN_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src}:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    return obj
+
+
+def build_sanitized(verbose: bool = True) -> Path:
+    """libsegscan_hip.so with host-side ASan + UBSan in _build_san/ (the normal build first
+    provides the kernel-only objects)."""
+    if not all((BUILD / (Path(s).stem + ".o")).exists() for s in SOURCES):
+        build(verbose)
+    SAN_BUILD.mkdir(exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=len(SAN_SOURCES)) as ex:
+        san_objs = list(ex.map(_compile_san, SAN_SOURCES))
+    objs = [o for o in (BUILD / (Path(s).stem + ".o") for s in SOURCES) if o.stem not in {Path(x).stem for x in SAN_SOURCES}]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-fsanitize=address,undefined", "-shared-libsan",
+           *map(str, objs), *map(str, san_objs), "-o", str(SAN_LIB)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {SAN_LIB}", file=sys.stderr)
+    return SAN_LIB
+
+
+def asan_runtime() -> Path:
+    """The clang ASan runtime that a sanitized library needs preloaded into Python."""
+    clang = Path(hipcc()).resolve().parent.parent / "llvm" / "bin" / "clang"
+    if not clang.exists():
+        clang = Path("/opt/rocm/llvm/bin/clang")
+    out = subprocess.run([str(clang), "-print-file-name=libclang_rt.asan-x86_64.so"], capture_output=True,
+                         text=True).stdout.strip()
+    return Path(out)
+
+
+def sanitized_env(base: dict | None = None) -> dict:
+    """Environment for a child Python that loads the sanitized library (leak checks off:
+    the interpreter itself keeps its allocations until exit)."""
+    env = dict(os.environ if base is None else base)
+    env["LD_PRELOAD"] = str(asan_runtime())
+    env["ASAN_OPTIONS"] = "detect_leaks=0:abort_on_error=0:halt_on_error=1"
+    env["UBSAN_OPTIONS"] = "print_stacktrace=1:halt_on_error=1"
+    env["SEGSCAN_LIB"] = str(SAN_LIB)
+    return env
+
+
 VENDOR_LIB = PKG / "libsegscan_vendor.so"
 
 
@@ -101,4 +162,7 @@ def build_vendor(verbose: bool = True) -> Path:
 
 
 if __name__ == "__main__":
-    build()
+    if "--sanitize" in sys.argv[1:]:
+        build_sanitized()
+    else:
+        build()

@@ -570,8 +570,8 @@ KernelFn pick_kernel(segscan_dtype_t t, int cf, int g) {
     case SEGSCAN_F32: return pick_cf<float>(cf, g);
     case SEGSCAN_BF16: return pick_cf<bf16_t>(cf, g);
     case SEGSCAN_F16: return pick_cf<f16_t>(cf, g);
+    default: return nullptr;
   }
-  return nullptr;
 }
 
 }  // namespace

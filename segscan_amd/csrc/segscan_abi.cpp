@@ -652,6 +652,7 @@ const char* segscan_status_string(segscan_status_t s) {
     case SEGSCAN_EUNSUPPORTED: return "SEGSCAN_EUNSUPPORTED: unsupported knob combination";
     case SEGSCAN_EHIP: return "SEGSCAN_EHIP: HIP runtime error";
     case SEGSCAN_EWORKSPACE: return "SEGSCAN_EWORKSPACE: workspace missing or too small";
+    default: break;
   }
   return "unknown segscan status";
 }
