@@ -83,7 +83,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-tuned", action="store_true", help="skip the comparison legs (for profiling)")
     ap.add_argument("--dump", type=str, default=None, help="write tuning results to this JSON file")
-    ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "fwd_bwd_csr_values", "segment_f32", "segment_bf16",
+    ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "fwd_bwd_csr_values", "values_permute", "segment_f32",
+                                           "segment_bf16",
                                            "batched_graph", "batched_graph_kernels"],
                     default=None,
                     help="run one comparison leg alone (for per-kernel profiles) and print its timing")
@@ -192,6 +193,14 @@ def only_leg(args):
         C = torch.empty((m.nrow, N), dtype=torch.float32, device=dev)
         ws = ops.spmm_workspace(A, N, torch.bfloat16, v, k)
         fn = lambda: ops.spmm(A, Bbf, variant=v, knobs=k, out=C, workspace=ws)  # noqa: E731
+    elif args.only_leg == "values_permute":
+        # the power-law matrix's values, kept in the transpose's order, permuted into A's order
+        T = ops.csr_transpose(A)
+        iperm = ops.invert_permutation(T.perm)
+        vals_t = A.vals.index_select(0, T.perm.long())
+        plan = ops.PermutePlan(iperm)
+        vals_a = torch.empty_like(vals_t)
+        fn = lambda: plan(vals_t, out=vals_a)  # noqa: E731
     elif args.only_leg in ("fwd_bwd", "fwd_bwd_csr_values"):
         dC = torch.from_numpy(rng.standard_normal((m.nrow, N)).astype(np.float32)).to(dev)
         T = ops.csr_transpose(A)
@@ -366,16 +375,30 @@ def main():
         ws_b = ops.spmm_backward_workspace(T, N, torch.float32)
 
         # learnable values kept in the transpose's order (the autograd layer's
-        # values_order="transpose"): the forward reads them through the inverse permutation,
-        # the backward reads them and writes dvals coalesced
+        # values_order="transpose"): each step permutes them into A's order with the two-pass
+        # plan and runs the forward on that copy; the backward reads them and writes dvals
+        # coalesced. fwd_bwd_vals_map reads them through the inverse permutation inside the
+        # forward SpMM instead.
         iperm = ops.invert_permutation(T.perm)
         vals_t = A.vals.index_select(0, T.perm.long())
         A_at = A.with_vals_map(vals_t, iperm)
+        perm_plan = ops.PermutePlan(iperm)
+        vals_a = torch.empty_like(vals_t)
+        A_perm = A.with_vals(vals_a)
 
         def fwd_bwd():
+            perm_plan(vals_t, out=vals_a)
+            ops.spmm(A_perm, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
+            ops.spmm_backward(T, None, dC, B32, dB=dB, dvals=dvals, workspace=ws_b, vals_t=vals_t,
+                              dvals_at_order=True)
+
+        def fwd_bwd_vals_map():
             ops.spmm(A_at, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
             ops.spmm_backward(T, None, dC, B32, dB=dB, dvals=dvals, workspace=ws_b, vals_t=vals_t,
                               dvals_at_order=True)
+
+        def permute_only():
+            perm_plan(vals_t, out=vals_a)
 
         def fwd_bwd_csr_values():
             # values in A's order: the backward reads them and writes dvals through perm
@@ -398,6 +421,8 @@ def main():
             ops.spmm_backward(Tn, A.vals, dC, B32, dB=dB, dvals=dvals, workspace=ws_b)
 
         fb_ms, _ = timed(fwd_bwd, 10, 3, kernel_events=False)
+        fbm_ms, _ = timed(fwd_bwd_vals_map, 10, 3, kernel_events=False)
+        perm_ms, _ = timed(permute_only, 20, 3, kernel_events=False)
         fbc_ms, _ = timed(fwd_bwd_csr_values, 10, 3, kernel_events=False)
         fbs_ms, _ = timed(fwd_bwd_separate, 10, 3, kernel_events=False)
         fbf_ms, _ = timed(fwd_bwd_fixed_vals, 10, 3, kernel_events=False)
@@ -430,6 +455,8 @@ def main():
             "hipsparse_f32_ms": vendor,
             "hipsparse_best_over_tuned_f32": None if best_vendor is None else round(best_vendor / ms_step, 3),
             "fwd_bwd_ms": round(fb_ms, 4),
+            "fwd_bwd_ms_vals_map": round(fbm_ms, 4),
+            "values_permute_ms": round(perm_ms, 4),
             "fwd_bwd_ms_csr_order_values": round(fbc_ms, 4),
             "fwd_bwd_ms_separate_kernels": round(fbs_ms, 4),
             "fwd_bwd_ms_fixed_vals": round(fbf_ms, 4),

@@ -296,6 +296,19 @@ segscan_status_t segscan_segment_gather(const float* ddst, int64_t ld_ddst, cons
 /* inv[perm[i]] = i for i < n (perm a permutation of [0, n)): the vals_map that reads values
  * kept in the transpose's order from A's entries. */
 segscan_status_t segscan_invert_permutation(const int32_t* perm, int64_t n, int32_t* inv, void* stream);
+/* A fixed permutation of per-entry values, out[i] = in[src[i]] (src a permutation of [0, n)),
+ * in two streaming passes instead of n random 4-byte reads: e.g. learnable values kept in the
+ * transpose's order, made into A's order once per step (src = segscan_invert_permutation(perm))
+ * so the forward SpMM reads them coalesced. Creation reads src back, checks it is a
+ * permutation (SEGSCAN_EINVAL otherwise), builds the plan's tables on the host and uploads
+ * them (8 bytes per value + a 4-byte-per-value staging buffer, allocated here); it
+ * synchronises `stream`. Runs of one plan must not overlap (they share the staging buffer);
+ * in and out must not alias. No counterpart in the reference (it has no backward). */
+typedef struct segscan_permute_plan* segscan_permute_plan_t;
+segscan_status_t segscan_permute_plan_create(const int32_t* src, int64_t n, void* stream,
+                                             segscan_permute_plan_t* plan);
+segscan_status_t segscan_permute_run(segscan_permute_plan_t plan, const float* in, float* out, void* stream);
+segscan_status_t segscan_permute_plan_destroy(segscan_permute_plan_t plan);
 /* counts[s] = #{e : index[e] == s}, s < dst_len (sorted index). */
 segscan_status_t segscan_segment_counts(const int32_t* index, int64_t E, int64_t dst_len, int32_t* counts,
                                         void* stream);

@@ -141,6 +141,9 @@ _SIGNATURES = {
     "segscan_probe_gather": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_int64, c_int64, c_void_p, c_int64,
                                        c_void_p]),
     "segscan_invert_permutation": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "segscan_permute_plan_create": (c_int32, [c_void_p, c_int64, c_void_p, POINTER(c_void_p)]),
+    "segscan_permute_run": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "segscan_permute_plan_destroy": (c_int32, [c_void_p]),
     "segscan_spmm_csr_noalloc": (c_int32, [POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p, c_int64,
                                            c_int32, POINTER(Knobs), c_uint32, c_void_p]),
     "segscan_segment_coo_describe": (c_int32, [c_int64, c_int64, c_int64, c_int32, c_int32, POINTER(Knobs), POINTER(LaunchInfo)]),

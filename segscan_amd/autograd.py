@@ -4,8 +4,10 @@ CPU.
 
 SpMM  C = A(vals) . B, A fixed pattern (rowptr, colind), vals and B learnable
 (vals in CSR order, or with values_order="transpose" in the transpose's order: the
-forward reads them through the inverse permutation and the fused backward reads them
-and writes their gradient coalesced, with no random access per entry):
+forward takes them into A's order with a two-pass permutation plan (or, with
+forward_values="vals_map", reads them through the inverse permutation inside the SpMM)
+and the fused backward reads them and writes their gradient coalesced, with no random
+access per entry):
     dB    = A^T . dC     CSR -> CSC transpose on the device (cached per pattern)
     dvals = SDDMM(dC, B) on A's pattern: dvals[p] = <dC[row(p)], B[colind[p]]>
     When dvals is wanted, both come from one fused walk over the transpose (one
@@ -27,9 +29,11 @@ class SpmmPattern:
     """A CSR pattern with its device transpose computed once."""
 
     def __init__(self, A: ops.CsrMatrix, variant="auto", knobs=None, variant_t="auto", knobs_t=None,
-                 fused: bool = True, knobs_bwd=None, values_order: str = "csr"):
+                 fused: bool = True, knobs_bwd=None, values_order: str = "csr", forward_values: str = "permute"):
         if values_order not in ("csr", "transpose"):
             raise ValueError("values_order must be 'csr' or 'transpose'")
+        if forward_values not in ("permute", "vals_map"):
+            raise ValueError("forward_values must be 'permute' or 'vals_map'")
         self.A = A
         self.values_order = values_order
         T = ops.csr_transpose(A)
@@ -40,8 +44,12 @@ class SpmmPattern:
         self.knobs_bwd = knobs_bwd  # fused backward shape (e.g. from autotune.tune_backward); None = built-in
         self._fused = {}
         self._vals_t, self._vals_t_key = None, None
-        # values kept in the transpose's order: the forward reads entry p's value at iperm[p]
+        # values kept in the transpose's order: the forward reads entry p's value at iperm[p],
+        # either from a copy permuted into A's order (two streaming passes) or inside the SpMM
         self.iperm = ops.invert_permutation(T.perm) if values_order == "transpose" else None
+        self.forward_values = forward_values
+        self.permute = (ops.PermutePlan(self.iperm) if values_order == "transpose" and forward_values == "permute"
+                        and A.nnz > 0 else None)
 
     def fused_ok(self, N: int) -> bool:
         """Whether the fused backward covers N columns (one group of lanes per dense row)."""
@@ -60,6 +68,8 @@ class SpmmPattern:
     def with_vals(self, vals: torch.Tensor) -> ops.CsrMatrix:
         """A with vals in this pattern's value order."""
         if self.values_order == "transpose":
+            if self.permute is not None:
+                return self.A.with_vals(self.permute(vals))
             return self.A.with_vals_map(vals, self.iperm)
         return self.A.with_vals(vals)
 

@@ -30,6 +30,7 @@ SOURCES = [
     "segscan_abi.cpp",
     "noalloc.hip",
     "batch_kernels.hip",
+    "permute.hip",
     "tune_table.hip",
     "mtx_loader.cpp",
 ]
@@ -90,7 +91,7 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
 # preloaded: see sanitized_env().
 SAN_BUILD = PKG / "_build_san"
 SAN_LIB = SAN_BUILD / "libsegscan_hip.so"
-SAN_SOURCES = ["segscan_abi.cpp", "mtx_loader.cpp", "tune_table.hip", "noalloc.hip", "batch_kernels.hip"]
+SAN_SOURCES = ["segscan_abi.cpp", "mtx_loader.cpp", "tune_table.hip", "noalloc.hip", "batch_kernels.hip", "permute.hip"]
 SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
              "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=undefined", "-g"]
 

@@ -307,6 +307,44 @@ def invert_permutation(perm: torch.Tensor, stream=None) -> torch.Tensor:
     return inv
 
 
+class PermutePlan:
+    """out[i] = x[src[i]] for a fixed permutation src of [0, n) (segscan_permute_plan_*): two
+    streaming passes over LDS-staged tiles instead of one random 4-byte read per value. For
+    values kept in the transpose's order, src = invert_permutation(T.perm) gives them in A's
+    order. Calls of one plan share its staging buffer: order them on one stream."""
+
+    def __init__(self, src: torch.Tensor, stream=None):
+        _require_device(src)
+        if src.dtype != torch.int32 or src.dim() != 1 or not src.is_contiguous() or src.numel() < 1:
+            raise ValueError("src must be a non-empty contiguous 1-D int32 tensor")
+        self.n = src.numel()
+        self.device = src.device
+        handle = ctypes.c_void_p()
+        _abi.check(_abi.load().segscan_permute_plan_create(_ptr(src), self.n, _stream(stream), ctypes.byref(handle)),
+                   "permute_plan_create")
+        self._h = handle
+
+    def __call__(self, x: torch.Tensor, out: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+        _require_device(x)
+        if x.dtype != torch.float32 or x.numel() != self.n or not x.is_contiguous():
+            raise ValueError(f"x must be contiguous float32 with {self.n} elements")
+        if out is None:
+            out = torch.empty_like(x)
+        elif out.dtype != torch.float32 or out.numel() != self.n or not out.is_contiguous():
+            raise ValueError(f"out must be contiguous float32 with {self.n} elements")
+        _abi.check(_abi.load().segscan_permute_run(self._h, _ptr(x), _ptr(out), _stream(stream)), "permute_run")
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _abi.load().segscan_permute_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
 def segment_counts(index: torch.Tensor, dst_len: int, stream=None) -> torch.Tensor:
     _require_device(index)
     lib = _abi.load()

@@ -341,3 +341,17 @@ def test_gather_probe_validation(lib):
     assert n.value == 1024  # 63 groups x 16 lanes, rounded up to whole workgroups of 256
     assert lib.segscan_probe_gather(0x1000, 10, 256, 0x2000, 1000, 16, 0x3000, 100, None) == _abi.EINVAL  # out short
     assert lib.segscan_probe_gather(0x1008, 10, 256, 0x2000, 1000, 16, 0x3000, 1024, None) == _abi.EINVAL  # misaligned
+
+
+def test_permute_plan_validation(lib):
+    """segscan_permute_plan_create / run reject bad arguments on the host, before any HIP call
+    (whether src is a permutation is checked after reading it back: GPU tests)."""
+    h = ctypes.c_void_p()
+    assert lib.segscan_permute_plan_create(0x1000, 0, None, byref(h)) == _abi.EINVAL
+    assert lib.segscan_permute_plan_create(0x1000, -5, None, byref(h)) == _abi.EINVAL
+    assert lib.segscan_permute_plan_create(0x1000, 1 << 31, None, byref(h)) == _abi.EINVAL
+    assert lib.segscan_permute_plan_create(None, 16, None, byref(h)) == _abi.EINVAL
+    assert lib.segscan_permute_plan_create(0x1000, 16, None, None) == _abi.EINVAL
+    assert lib.segscan_last_error().decode() != ""
+    assert lib.segscan_permute_run(None, 0x1000, 0x2000, None) == _abi.EINVAL
+    assert lib.segscan_permute_plan_destroy(None) == _abi.OK

@@ -21,6 +21,8 @@ SUITES = ["tests/test_abi_cpu.py", "tests/test_mtx.py", "tests/test_tune_table.p
 def test_host_code_clean_under_asan_and_ubsan():
     if not build.SAN_LIB.exists():
         pytest.skip(f"{build.SAN_LIB} not built (python -m segscan_amd.build --sanitize)")
+    if build.LIB.exists() and build.SAN_LIB.stat().st_mtime < build.LIB.stat().st_mtime:
+        pytest.skip(f"{build.SAN_LIB} is older than {build.LIB.name} (python -m segscan_amd.build --sanitize)")
     env = build.sanitized_env()
     env.pop("PYTEST_XDIST_WORKER", None)
     r = subprocess.run([sys.executable, "-m", "pytest", *SUITES, "-m", "not gpu", "-q", "-s", "-x",
