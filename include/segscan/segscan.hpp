@@ -18,7 +18,8 @@
 // the reference harness does), store variants overwrite; the launchers are
 // asynchronous on the default stream, except that the segment_coo family reads
 // the last index value back to size dst (the reference's segment_coo launchers
-// synchronise too, segscan.cuh:179). Differences: failures throw
+// synchronise too, segscan.cuh:179); an optional trailing dst_len argument (the
+// row count of dst) skips that read, for hipGraph capture. Differences: failures throw
 // std::runtime_error (the reference's SpMM launchers check nothing; its
 // segment_coo launchers exit()), values are read as float in every variant, and
 // scratch space (carries, ebpr tile rows) is cached in the descriptor.
@@ -173,14 +174,20 @@ inline void* seg_workspace(size_t bytes) {
   return ws;
 }
 
+// dst_len < 0: read the last index back (a blocking copy, as the reference's launchers take
+// no output length); dst_len >= 0: the caller's row count of dst, no host round trip (the
+// form to use under hipGraph capture).
 template <class ValueType, class IndexType>
 void segment(const ValueType* src, const IndexType* index, int nnz, int N, ValueType* dst, segscan_seg_variant_t v,
-             const segscan_knobs_t* k) {
+             const segscan_knobs_t* k, int dst_rows) {
   static_assert(sizeof(ValueType) == 4 && sizeof(IndexType) == 4, "float values, int32 index");
   if (nnz <= 0 || N <= 0) return;
-  IndexType last = 0;
-  check_hip(hipMemcpy(&last, index + nnz - 1, sizeof(IndexType), hipMemcpyDeviceToHost), "read last index");
-  const int64_t dst_len = static_cast<int64_t>(last) + 1;
+  int64_t dst_len = dst_rows;
+  if (dst_rows < 0) {
+    IndexType last = 0;
+    check_hip(hipMemcpy(&last, index + nnz - 1, sizeof(IndexType), hipMemcpyDeviceToHost), "read last index");
+    dst_len = static_cast<int64_t>(last) + 1;
+  }
   size_t bytes = 0;
   check_status(segscan_segment_coo_workspace_size(nnz, N, dst_len, SEGSCAN_F32, v, k, &bytes), "segment workspace");
   void* ws = bytes ? seg_workspace(bytes) : nullptr;
@@ -244,43 +251,43 @@ void csrspmm_seqreduce_rowbalance_cg(SpMatCsrDescr_t<Index, DType>& A, const int
 
 template <typename ValueType, typename IndexType, int group_factor, int thread_per_block, int tile_factor,
           int block_numer, int block_denom>
-void segment_coo(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst) {
+void segment_coo(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst, const int dst_len = -1) {
   const segscan_knobs_t k = detail::knobs(group_factor, tile_factor, thread_per_block, block_numer, block_denom);
-  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_PR, &k);
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_PR, &k, dst_len);
 }
 template <typename ValueType, typename IndexType, int group_factor, int thread_per_block, int tile_factor,
           int blockDimY>
-void segment_coo_new(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst) {
+void segment_coo_new(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst, const int dst_len = -1) {
   segscan_knobs_t k = detail::knobs(group_factor, tile_factor, thread_per_block, 1, 1);
   k.block_dim_y = blockDimY;
-  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_PR_NEW, &k);
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_PR_NEW, &k, dst_len);
 }
 template <typename ValueType, typename IndexType, int group_factor, int thread_per_block, int CoarsenFactor,
           int ThreadNz, int block_numer, int block_denom>
-void segment_coo_sr(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst) {
+void segment_coo_sr(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst, const int dst_len = -1) {
   const segscan_knobs_t k = detail::knobs(group_factor, 0, thread_per_block, block_numer, block_denom, CoarsenFactor,
                                           ThreadNz, SEGSCAN_WRITE_ATOMIC);
-  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_SR, &k);
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_SR, &k, dst_len);
 }
 template <typename ValueType, typename IndexType, int group_factor, int thread_per_block, int CoarsenFactor,
           int ThreadNz, int block_numer, int block_denom>
 void segment_coo_noshmem_sr(const ValueType* src, const IndexType* index, const int nnz, const int N,
-                            ValueType* dst) {
+                            ValueType* dst, const int dst_len = -1) {
   const segscan_knobs_t k = detail::knobs(group_factor, 0, thread_per_block, block_numer, block_denom, CoarsenFactor,
                                           ThreadNz, SEGSCAN_WRITE_ATOMIC);
-  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_NOSHMEM_SR, &k);
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_NOSHMEM_SR, &k, dst_len);
 }
 template <typename ValueType, typename IndexType, int group_factor, int thread_per_block, int CoarsenFactor,
           int ThreadNz, int block_numer, int block_denom>
 void segment_coo_noshmem_lessatom_sr(const ValueType* src, const IndexType* index, const int nnz, const int N,
-                                     ValueType* dst) {
+                                     ValueType* dst, const int dst_len = -1) {
   const segscan_knobs_t k = detail::knobs(group_factor, 0, thread_per_block, block_numer, block_denom, CoarsenFactor,
                                           ThreadNz, SEGSCAN_WRITE_LESSATOM);
-  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_LESSATOM_SR, &k);
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_LESSATOM_SR, &k, dst_len);
 }
 template <typename ValueType, typename IndexType, int CoarsenFactor, int ThreadNz, int DtileSize, int DtileNum,
           int BlockDimY>
-void segment_coo_sorted_sr(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst) {
+void segment_coo_sorted_sr(const ValueType* src, const IndexType* index, const int nnz, const int N, ValueType* dst, const int dst_len = -1) {
   segscan_knobs_t k = detail::knobs(0, 0, 0, 1, 1, CoarsenFactor, ThreadNz, SEGSCAN_WRITE_LESSATOM);
   k.group_factor = 0;
   k.tile_factor = 0;
@@ -288,7 +295,7 @@ void segment_coo_sorted_sr(const ValueType* src, const IndexType* index, const i
   k.dtile_size = DtileSize;
   k.dtile_num = DtileNum;
   k.block_dim_y = BlockDimY;
-  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_SORTED_SR, &k);
+  detail::segment(src, index, nnz, N, dst, SEGSCAN_SEG_SORTED_SR, &k, dst_len);
 }
 
 // ---- Matrix Market input (test-cg/include/dataloader/dataloader.hpp:22-192) ----------------

@@ -213,6 +213,13 @@ static void segment_suite() {
   run("segment_coo_sorted_sr<float,int,2,4,8,2,16>", [&](float* s, int* ix, float* d) {
     segscan::segment_coo_sorted_sr<float, int, 2, 4, 8, 2, 16>(s, ix, nnz, N, d);
   });
+  // the same launchers given dst's row count: no read-back of the last index
+  run("segment_coo_sorted_sr<float,int,2,4,8,2,16>(dst_len)", [&](float* s, int* ix, float* d) {
+    segscan::segment_coo_sorted_sr<float, int, 2, 4, 8, 2, 16>(s, ix, nnz, N, d, range);
+  });
+  run("segment_coo<float,int,5,256,5,1,16>(dst_len)", [&](float* s, int* ix, float* d) {
+    segscan::segment_coo<float, int, 5, 256, 5, 1, 16>(s, ix, nnz, N, d, range);
+  });
 }
 
 // Usage: spmm_check                      synthetic suites
