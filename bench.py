@@ -421,6 +421,17 @@ def main():
             ops.spmm_backward(Tn, A.vals, dC, B32, dB=dB, dvals=dvals, workspace=ws_b)
 
         fb_ms, _ = timed(fwd_bwd, 10, 3, kernel_events=False)
+        # the same step captured as one hipGraph (launch gaps between its six kernels shrink)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            fwd_bwd()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        step_graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(step_graph):
+            fwd_bwd()
+        fbg_ms, _ = timed(step_graph.replay, 10, 3, kernel_events=False)
         fbm_ms, _ = timed(fwd_bwd_vals_map, 10, 3, kernel_events=False)
         perm_ms, _ = timed(permute_only, 20, 3, kernel_events=False)
         fbc_ms, _ = timed(fwd_bwd_csr_values, 10, 3, kernel_events=False)
@@ -456,6 +467,7 @@ def main():
             "hipsparse_best_over_tuned_f32": None if best_vendor is None else round(best_vendor / ms_step, 3),
             "fwd_bwd_ms": round(fb_ms, 4),
             "fwd_bwd_ms_vals_map": round(fbm_ms, 4),
+            "fwd_bwd_ms_hipgraph": round(fbg_ms, 4),
             "values_permute_ms": round(perm_ms, 4),
             "fwd_bwd_ms_csr_order_values": round(fbc_ms, 4),
             "fwd_bwd_ms_separate_kernels": round(fbs_ms, 4),
