@@ -75,6 +75,7 @@ PLAN_CASES = [
     (torch.bfloat16, 100, None),                               # 64 lanes x 2 columns, partial last lane
     (torch.float32, 128, dict(block_numer=1, block_denom=16)),  # few workgroups: many items each
     (torch.float16, 33, dict(coarsen_factor=1)),
+    (torch.float32, 64, dict(block_numer=1, block_denom=1024)),  # a grid under the XCD count: items in order
 ]
 
 
