@@ -87,8 +87,93 @@ __global__ void __launch_bounds__(1024) rbsr_kernel(RbsrParams p) {
   }
 }
 
+// The same walk when a row's lanes are a power of two in [8, 64] (so rows never straddle a
+// wave): the row's lanes load the column and value of tpr consecutive entries at once (one
+// each) and hand them round with shuffles, instead of every lane loading every entry's
+// metadata; the next tile's metadata is in flight while the current tile's dense rows are
+// gathered, RB at a time. Entries are summed in the same order as rbsr_kernel.
+template <typename VT, int CF>
+__global__ void __launch_bounds__(1024) rbsr_coop_kernel(RbsrParams p) {
+  constexpr int RB = 8;  // dense-row loads per batch
+  const int L = threadIdx.x;
+  const int tpr = p.tpr;
+  const int yr = L / tpr;
+  const int xv = L - yr * tpr;
+  if (yr >= p.rows_per_block) return;  // whole rows (tpr divides the wave)
+  const int rbase = __lane_id() - xv;  // first lane of this row in the wave
+  const int col0 = blockIdx.y * p.slice + xv * CF;
+  const int nv = min(CF, p.N - col0);  // <= 0: the lane only carries metadata
+  const bool full = nv == CF;
+  const VT* __restrict__ B = static_cast<const VT*>(p.B);
+  const uint32_t ldb32 = static_cast<uint32_t>(p.ldb);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * p.rows_per_block;
+  auto meta = [&](int e, int end, int& k, float& v) {
+    k = 0;
+    v = 0.0f;
+    if (e < end) {
+      k = p.colind[e];
+      v = p.vals ? p.vals[p.vmap ? p.vmap[e] : e] : 1.0f;
+    }
+  };
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * p.rows_per_block + yr; r < p.rows_total; r += stride) {
+    const int start = p.rowptr[r];
+    const int end = p.rowptr[r + 1];
+    float acc[CF];
+#pragma unroll
+    for (int i = 0; i < CF; ++i) acc[i] = 0.0f;
+    int mk;
+    float mv;
+    meta(start + xv, end, mk, mv);
+    for (int q = start; q < end; q += tpr) {
+      const int cnt = min(tpr, end - q);
+      int nk;
+      float nvv;
+      meta(q + tpr + xv, end, nk, nvv);  // next tile, in flight during this one
+      for (int b = 0; b < cnt; b += RB) {
+        RawCols<VT, CF> raw[RB];
+        int kk[RB];
+        float vv[RB];
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+          kk[u] = __shfl(mk, rbase + b + u);
+          vv[u] = __shfl(mv, rbase + b + u);
+          // slots past the tile read dense row 0 (always valid) and are never summed:
+          // one branch-free batch keeps all RB loads in flight together
+          if (b + u >= cnt) kk[u] = 0;
+        }
+        if (full) {
+#pragma unroll
+          for (int u = 0; u < RB; ++u)
+            load_raw<VT, CF>(B + static_cast<uint64_t>(static_cast<uint32_t>(kk[u])) * ldb32 + col0, raw[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+          if (b + u < cnt && nv > 0) {
+            float bb[CF];
+            if (full) widen<VT, CF>(raw[u], bb);
+            else load_cols_partial<VT, CF>(B + static_cast<int64_t>(kk[u]) * p.ldb + col0, nv, bb);
+#pragma unroll
+            for (int i = 0; i < CF; ++i) acc[i] += vv[u] * bb[i];
+          }
+        }
+      }
+      mk = nk;
+      mv = nvv;
+    }
+    if (nv > 0) store_cols<CF>(p.C + r * p.ldc + col0, nv, acc, p.accumulate != 0);
+  }
+}
+
 template <typename VT, int CF>
 static hipError_t launch_rbsr_t(const RbsrParams& p, const Launch& l) {
+  // With 8 columns per lane the widened rows of a batch take the cooperative kernel to
+  // 113-120 VGPRs (4 waves per SIMD) and it is no faster; rbsr_kernel keeps that shape.
+  if constexpr (CF <= 4) {
+    if (p.tpr >= 8 && p.tpr <= 64 && (p.tpr & (p.tpr - 1)) == 0) {
+      rbsr_coop_kernel<VT, CF><<<l.grid, l.block, 0, l.stream>>>(p);
+      return hipGetLastError();
+    }
+  }
   rbsr_kernel<VT, CF><<<l.grid, l.block, 0, l.stream>>>(p);
   return hipGetLastError();
 }

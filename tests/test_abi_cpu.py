@@ -266,6 +266,16 @@ def test_describe_auto_16bit_takes_two_columns_per_lane(lib):
     assert (info.group_size, info.coarsen_factor) == (64, 8)  # widened only when N needs more than 64 lanes
 
 
+def test_describe_auto_even_rows_take_8_byte_slices(lib):
+    """Even rows: row-balanced sequential reduction over 64-column slices with at most 4
+    columns per lane, for fp32 and 16-bit B alike (16 lanes per row from N = 64 on)."""
+    for dtype in (_abi.F32, _abi.BF16, _abi.F16):
+        for N, cf in ((64, 4), (128, 4), (32, 4), (20, 4), (34, 2), (33, 1)):
+            info = _describe(lib, 120000, 2520000, N, _abi.SPMM_AUTO, dtype=dtype, max_row=24)
+            assert info.variant == _abi.SPMM_RBSR, (dtype, N)
+            assert info.coarsen_factor == cf, (dtype, N, info.coarsen_factor)
+
+
 def test_workspace_size_query(lib):
     d = _desc(nrow=1000, ncol=1000, nnz=5000)
     n = c_size_t(0)
