@@ -141,7 +141,7 @@ def spmm_candidates(N: int, dtype: torch.dtype, stats: dict) -> list[tuple[str, 
         cands.append(("rbpr", dict(group_factor=gfac, tile_factor=tfac, thread_per_block=256,
                                    block_numer=1, block_denom=2)))
     # rb_sr: columns per lane x slice width
-    for c in sorted({1, 2, cf}):
+    for c in sorted({1, 2, min(4, cf), cf}):
         for tfac in (5, 6):
             cands.append(("rbsr", dict(coarsen_factor=c, tile_factor=tfac, thread_per_block=256)))
     # eb_sr: a group of lanes over one dense row (128-bit loads, or 64/32-bit with

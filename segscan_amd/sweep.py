@@ -100,7 +100,7 @@ def spmm_actions(N: int, dtype_bytes: int = 4, families: Iterable[str] = FAMILIE
                     grid.append(dict(group_factor=gf, thread_per_block=256, tile_factor=tf, block_numer=1,
                                      block_denom=bd))
         elif fam == "rbsr":
-            for c in sorted({1, 2, cf}):
+            for c in sorted({1, 2, min(4, cf), cf}):
                 for tf in (5, 6):
                     grid.append(dict(coarsen_factor=c, thread_per_block=256, tile_factor=tf))
         else:

@@ -268,9 +268,10 @@ def test_describe_auto_16bit_takes_two_columns_per_lane(lib):
 
 def test_describe_auto_even_rows_take_8_byte_slices(lib):
     """Even rows: row-balanced sequential reduction over 64-column slices with at most 4
-    columns per lane, for fp32 and 16-bit B alike (16 lanes per row from N = 64 on)."""
+    columns per lane, for fp32 and 16-bit B alike (16 lanes per row from N = 64 on), narrower
+    until a row has at least 8 lanes."""
     for dtype in (_abi.F32, _abi.BF16, _abi.F16):
-        for N, cf in ((64, 4), (128, 4), (32, 4), (20, 4), (34, 2), (33, 1)):
+        for N, cf in ((64, 4), (128, 4), (32, 4), (20, 2), (16, 2), (34, 2), (33, 1), (24, 2)):
             info = _describe(lib, 120000, 2520000, N, _abi.SPMM_AUTO, dtype=dtype, max_row=24)
             assert info.variant == _abi.SPMM_RBSR, (dtype, N)
             assert info.coarsen_factor == cf, (dtype, N, info.coarsen_factor)
