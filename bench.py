@@ -392,6 +392,28 @@ def main():
             ops.spmm_backward(T, None, dC, B32, dB=dB, dvals=dvals, workspace=ws_b, vals_t=vals_t,
                               dvals_at_order=True)
 
+        # the same step with B stored in bf16 (the tuned bf16 forward; the backward reads B as
+        # stored under an fp32 dC, dB in fp32)
+        ws_fb = ops.spmm_workspace(A, N, torch.bfloat16, tuned[torch.bfloat16][0], tuned[torch.bfloat16][1])
+        ws_bb = ops.spmm_backward_workspace(T, N, torch.float32)
+
+        def fwd_bwd_bf16():
+            perm_plan(vals_t, out=vals_a)
+            ops.spmm(A_perm, Bbf, variant=tuned[torch.bfloat16][0], knobs=tuned[torch.bfloat16][1], out=C,
+                     workspace=ws_fb)
+            ops.spmm_backward(T, None, dC, Bbf, dB=dB, dvals=dvals, workspace=ws_bb, vals_t=vals_t,
+                              dvals_at_order=True)
+
+        # fp32 with the reference's plain launcher wherever it has one: plain ebpr forward,
+        # plain ebpr on the transpose for dB, and the SDDMM kernel for dvals (no plain form)
+        ws_pf = ops.spmm_workspace(A, N, torch.float32, "plain_ebpr")
+        ws_pt = ops.spmm_workspace(Tt, N, torch.float32, "plain_ebpr")
+
+        def fwd_bwd_plain():
+            ops.spmm(A, B32, variant="plain_ebpr", out=C, workspace=ws_pf)
+            ops.spmm(Tt, dC, variant="plain_ebpr", out=dB, workspace=ws_pt)
+            ops.sddmm(A, dC, B32, out=dvals)
+
         def fwd_bwd_vals_map():
             ops.spmm(A_at, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
             ops.spmm_backward(T, None, dC, B32, dB=dB, dvals=dvals, workspace=ws_b, vals_t=vals_t,
@@ -433,6 +455,8 @@ def main():
             fwd_bwd()
         fbg_ms, _ = timed(step_graph.replay, 10, 3, kernel_events=False)
         fbm_ms, _ = timed(fwd_bwd_vals_map, 10, 3, kernel_events=False)
+        fbb_ms, _ = timed(fwd_bwd_bf16, 10, 3, kernel_events=False)
+        fbp_ms, _ = timed(fwd_bwd_plain, 5, 2, kernel_events=False)
         perm_ms, _ = timed(permute_only, 20, 3, kernel_events=False)
         fbc_ms, _ = timed(fwd_bwd_csr_values, 10, 3, kernel_events=False)
         fbs_ms, _ = timed(fwd_bwd_separate, 10, 3, kernel_events=False)
@@ -468,6 +492,9 @@ def main():
             "fwd_bwd_ms": round(fb_ms, 4),
             "fwd_bwd_ms_vals_map": round(fbm_ms, 4),
             "fwd_bwd_ms_hipgraph": round(fbg_ms, 4),
+            "fwd_bwd_ms_bf16": round(fbb_ms, 4),
+            "fwd_bwd_ms_plain_f32": round(fbp_ms, 4),
+            "fwd_bwd_speedup_bf16_vs_plain_f32": round(fbp_ms / fbb_ms, 2),
             "values_permute_ms": round(perm_ms, 4),
             "fwd_bwd_ms_csr_order_values": round(fbc_ms, 4),
             "fwd_bwd_ms_separate_kernels": round(fbs_ms, 4),
