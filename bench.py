@@ -84,7 +84,7 @@ def parse():
     ap.add_argument("--only-tuned", action="store_true", help="skip the comparison legs (for profiling)")
     ap.add_argument("--dump", type=str, default=None, help="write tuning results to this JSON file")
     ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "fwd_bwd_csr_values", "values_permute", "segment_f32",
-                                           "segment_bf16",
+                                           "segment_bf16", "banded_f32", "banded_bf16", "banded_f16",
                                            "batched_graph", "batched_graph_kernels"],
                     default=None,
                     help="run one comparison leg alone (for per-kernel profiles) and print its timing")
@@ -173,6 +173,17 @@ def only_leg(args):
     dev = torch.device("cuda:0")
     N = args.N
     rng = np.random.default_rng(0)
+    if args.only_leg.startswith("banded_"):
+        # the FEM-like banded matrix (120000^2 of 3x3 blocks, ~21 nnz per row), AUTO (the
+        # cooperative row walk), B stored as fp32 / bf16 / fp16
+        dt = {"banded_f32": torch.float32, "banded_bf16": torch.bfloat16, "banded_f16": torch.float16}[args.only_leg]
+        mb = synth.banded_block_csr(seed=0)
+        Ab = mb.to_device(dev).measured()
+        Bb = torch.from_numpy(rng.uniform(-1, 1, size=(mb.ncol, N)).astype(np.float32)).to(dev).to(dt)
+        Cb = torch.empty((mb.nrow, N), dtype=torch.float32, device=dev)
+        wsb = ops.spmm_workspace(Ab, N, dt, "auto")
+        _time_leg(args, lambda: ops.spmm(Ab, Bb, "auto", out=Cb, workspace=wsb))
+        return
     if args.only_leg in ("batched_graph", "batched_graph_kernels"):
         csr, offsets = synth.batched_graphs(seed=0)
         gb = batch.make_batch(csr, offsets, dev)

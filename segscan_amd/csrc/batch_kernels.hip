@@ -44,7 +44,6 @@
 namespace segscan {
 
 constexpr int kBatchTpb = 256;
-constexpr int kXcds = 8;  // XCDs of an MI355X (gfx950), each with its own L2
 
 struct BatchParams {
   int M;                   // nodes (rows = columns of A)

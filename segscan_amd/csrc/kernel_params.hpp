@@ -12,6 +12,10 @@ namespace segscan {
 
 enum class VType : int { F32 = 0, BF16 = 1, F16 = 2 };
 
+// XCDs of an MI355X (gfx950), each with its own L2. Workgroups are dispatched to them round
+// robin: workgroup w runs on XCD w % kXcds.
+constexpr int kXcds = 8;
+
 // Owner write policy: empty output rows are tracked in blocks of kZeroRows rows, one
 // flag byte per block (owner_empty_run, carry_fixup_kernel).
 constexpr int kZeroRowsLog = 4;
