@@ -117,28 +117,6 @@ __device__ __forceinline__ void load_f32_cols(const float* __restrict__ p, int n
   }
 }
 
-// One step of a reduce-scatter over an aligned group of G lanes: lanes with bit m clear
-// return a + a' and lanes with bit m set b + b' (primes: the partner lane l ^ m). Fed
-// the U per-entry partial sums of a batch in pairs, level by level as they are produced
-// (a streaming butterfly, at most log2 U pending values), it leaves lane l holding the
-// group total of entry sum_h bit(l, G >> (h + 1)) << h.
-template <int G>
-__device__ __forceinline__ float butterfly(float a, float b, int m) {
-  const bool hi = (__lane_id() & m) != 0;
-  const float send = hi ? a : b;
-  const float keep = hi ? b : a;
-  return keep + __shfl_xor(send, m, G);
-}
-
-// Entry of a batch whose group total the butterfly leaves in lane gl.
-template <int G, int U>
-__device__ __forceinline__ int butterfly_entry(int gl) {
-  constexpr int H = __builtin_ctz(U);
-  int e = 0;
-#pragma unroll
-  for (int h = 0; h < H; ++h) e |= ((gl & (G >> (h + 1))) ? 1 : 0) << h;
-  return e;
-}
 
 // Waves per SIMD the register allocator is held to: with up to 8 bytes of a dense row
 // per lane the kernel fits 128 VGPRs with a few spills, and 4 waves beat 3 unspilled

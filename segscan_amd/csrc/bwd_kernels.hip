@@ -212,15 +212,33 @@ __device__ __forceinline__ void bwd_walk(const BwdParams& p, int32_t* bwd_smem) 
           }
         }
         if (want_dv) {
+          // The batch's U dot products reduced across the group by a butterfly (U - 1 pair
+          // exchanges plus log2(G / U) final ones, instead of log2 G shifts per entry);
+          // lane l then holds the total of entry butterfly_entry(l) and the lanes that
+          // hold one store it.
+          constexpr int kLevels = __builtin_ctz(U);
+          float lvl[kLevels + 1];
 #pragma unroll
           for (int u = 0; u < U; ++u) {
-            float t[1] = {part[u]};
-            group_reduce_to_first<G, 1>(t);
-            // non-temporal: a scattered 4-byte store would otherwise hold an L2 line
-            // that the dC gathers reuse (power-law 262144^2, N = 64: 218 -> 196 us with
-            // At-order values); non-temporal value loads measured slower and are not used
-            if (g_lane == 0 && pp + u < cnt) __builtin_nontemporal_store(t[0], p.dvals + s_p[sbase + pp + u]);
+            float d = part[u];
+#pragma unroll
+            for (int h = 0; h <= kLevels; ++h) {
+              if (h == kLevels || ((u >> h) & 1) == 0) {
+                lvl[h] = d;
+                break;
+              }
+              d = butterfly<G>(lvl[h], d, G >> (h + 1));
+            }
           }
+          float tot = lvl[kLevels];
+#pragma unroll
+          for (int m = (G >> (kLevels + 1)); m >= 1; m >>= 1) tot += __shfl_xor(tot, m, G);
+          const int eu = butterfly_entry<G, U>(g_lane);
+          // non-temporal: a scattered 4-byte store would otherwise hold an L2 line that the
+          // dC gathers reuse (power-law 262144^2, N = 64: 218 -> 196 us with At-order
+          // values); non-temporal value loads measured slower and are not used
+          if ((g_lane & ((G >> kLevels) - 1)) == 0 && pp + eu < cnt)
+            __builtin_nontemporal_store(tot, p.dvals + s_p[sbase + pp + eu]);
         }
       }
       __builtin_amdgcn_wave_barrier();  // LDS slots are rewritten by the next tile
