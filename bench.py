@@ -433,8 +433,21 @@ def main():
         def permute_only():
             perm_plan(vals_t, out=vals_a)
 
+        # values in A's order (the autograd layer's default): two permutation plans take them
+        # into the transpose's order and dvals back, around the coalesced backward;
+        # fwd_bwd_csr_values_in_walk reads and writes them through perm inside the walk
+        perm_to_t = ops.PermutePlan(T.perm)
+        vals_tt = torch.empty_like(vals_t)
+        dvals_t = torch.empty_like(dvals)
+
         def fwd_bwd_csr_values():
-            # values in A's order: the backward reads them and writes dvals through perm
+            ops.spmm(A, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
+            perm_to_t(A.vals, out=vals_tt)
+            ops.spmm_backward(T, None, dC, B32, dB=dB, dvals=dvals_t, workspace=ws_b, vals_t=vals_tt,
+                              dvals_at_order=True)
+            perm_plan(dvals_t, out=dvals)
+
+        def fwd_bwd_csr_values_in_walk():
             ops.spmm(A, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
             ops.spmm_backward(T, A.vals, dC, B32, dB=dB, dvals=dvals, workspace=ws_b)
 
@@ -470,6 +483,7 @@ def main():
         fbp_ms, _ = timed(fwd_bwd_plain, 5, 2, kernel_events=False)
         perm_ms, _ = timed(permute_only, 20, 3, kernel_events=False)
         fbc_ms, _ = timed(fwd_bwd_csr_values, 10, 3, kernel_events=False)
+        fbw_ms, _ = timed(fwd_bwd_csr_values_in_walk, 10, 3, kernel_events=False)
         fbs_ms, _ = timed(fwd_bwd_separate, 10, 3, kernel_events=False)
         fbf_ms, _ = timed(fwd_bwd_fixed_vals, 10, 3, kernel_events=False)
         fbt_ms, _ = timed(fwd_bwd_with_transpose, 5, 2, kernel_events=False)
@@ -508,6 +522,7 @@ def main():
             "fwd_bwd_speedup_bf16_vs_plain_f32": round(fbp_ms / fbb_ms, 2),
             "values_permute_ms": round(perm_ms, 4),
             "fwd_bwd_ms_csr_order_values": round(fbc_ms, 4),
+            "fwd_bwd_ms_csr_order_values_in_walk": round(fbw_ms, 4),
             "fwd_bwd_ms_separate_kernels": round(fbs_ms, 4),
             "fwd_bwd_ms_fixed_vals": round(fbf_ms, 4),
             "fwd_bwd_ms_incl_transpose": round(fbt_ms, 4),

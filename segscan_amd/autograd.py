@@ -7,7 +7,9 @@ SpMM  C = A(vals) . B, A fixed pattern (rowptr, colind), vals and B learnable
 forward takes them into A's order with a two-pass permutation plan (or, with
 forward_values="vals_map", reads them through the inverse permutation inside the SpMM)
 and the fused backward reads them and writes their gradient coalesced, with no random
-access per entry):
+access per entry; for CSR-order values the backward permutes them into the transpose's
+order and the gradient back with two such plans, or with backward_values="perm" reads
+and writes them through perm inside the walk):
     dB    = A^T . dC     CSR -> CSC transpose on the device (cached per pattern)
     dvals = SDDMM(dC, B) on A's pattern: dvals[p] = <dC[row(p)], B[colind[p]]>
     When dvals is wanted, both come from one fused walk over the transpose (one
@@ -29,11 +31,14 @@ class SpmmPattern:
     """A CSR pattern with its device transpose computed once."""
 
     def __init__(self, A: ops.CsrMatrix, variant="auto", knobs=None, variant_t="auto", knobs_t=None,
-                 fused: bool = True, knobs_bwd=None, values_order: str = "csr", forward_values: str = "permute"):
+                 fused: bool = True, knobs_bwd=None, values_order: str = "csr", forward_values: str = "permute",
+                 backward_values: str = "permute"):
         if values_order not in ("csr", "transpose"):
             raise ValueError("values_order must be 'csr' or 'transpose'")
         if forward_values not in ("permute", "vals_map"):
             raise ValueError("forward_values must be 'permute' or 'vals_map'")
+        if backward_values not in ("permute", "perm"):
+            raise ValueError("backward_values must be 'permute' or 'perm'")
         self.A = A
         self.values_order = values_order
         T = ops.csr_transpose(A)
@@ -50,6 +55,20 @@ class SpmmPattern:
         self.forward_values = forward_values
         self.permute = (ops.PermutePlan(self.iperm) if values_order == "transpose" and forward_values == "permute"
                         and A.nnz > 0 else None)
+        # values in A's order: the fused backward takes them into the transpose's order and
+        # its dvals back with two permutation plans (streaming passes) instead of one random
+        # read and one random write per entry inside the walk; built on first use
+        self.backward_values = backward_values
+        self._to_t = self._to_csr = None
+
+    def csr_permute_plans(self):
+        """(to the transpose's order, back to A's order) permutation plans, or None."""
+        if self.values_order != "csr" or self.backward_values != "permute" or self.A.nnz == 0:
+            return None
+        if self._to_t is None:
+            self._to_t = ops.PermutePlan(self.T.perm)  # x_t[q] = x[perm[q]]
+            self._to_csr = ops.PermutePlan(ops.invert_permutation(self.T.perm))  # x[p] = x_t[iperm[p]]
+        return self._to_t, self._to_csr
 
     def fused_ok(self, N: int) -> bool:
         """Whether the fused backward covers N columns (one group of lanes per dense row)."""
@@ -121,6 +140,12 @@ class SpmmFunction(torch.autograd.Function):
                 # values and their gradient in the walk's own order: no perm at all
                 dB, dvals = ops.spmm_backward(pattern.T, None, dC, Bs, want_dB=want_b, knobs=pattern.knobs_bwd,
                                               vals_t=v32, dvals_at_order=True)
+            elif pattern.csr_permute_plans() is not None:
+                # values taken into the walk's order and dvals back by two streaming passes
+                to_t, to_csr = pattern.csr_permute_plans()
+                dB, dvals_t = ops.spmm_backward(pattern.T, None, dC, Bs, want_dB=want_b, knobs=pattern.knobs_bwd,
+                                                vals_t=to_t(v32), dvals_at_order=True)
+                dvals = to_csr(dvals_t)
             else:
                 # values read through the permutation inside the kernel
                 dB, dvals = ops.spmm_backward(pattern.T, v32, dC, Bs, want_dB=want_b, knobs=pattern.knobs_bwd)
