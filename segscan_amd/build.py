@@ -114,8 +114,9 @@ def build_sanitized(verbose: bool = True) -> Path:
     with cf.ThreadPoolExecutor(max_workers=len(SAN_SOURCES)) as ex:
         san_objs = list(ex.map(_compile_san, SAN_SOURCES))
     objs = [o for o in (BUILD / (Path(s).stem + ".o") for s in SOURCES) if o.stem not in {Path(x).stem for x in SAN_SOURCES}]
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-fsanitize=address,undefined", "-shared-libsan",
-           *map(str, objs), *map(str, san_objs), "-o", str(SAN_LIB)]
+    # host-side runtimes only: -fno-gpu-sanitize keeps the device link uninstrumented
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-fsanitize=address,undefined", "-fno-gpu-sanitize",
+           "-shared-libsan", *map(str, objs), *map(str, san_objs), "-o", str(SAN_LIB)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
