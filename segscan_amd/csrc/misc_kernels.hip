@@ -188,11 +188,15 @@ radix_upsweep_kernel(const int32_t* __restrict__ keys, int64_t nnz, int shift, i
   for (int b = threadIdx.x; b < buckets; b += kRadixThreads) hist[static_cast<int64_t>(b) * ntiles + tile] = cnt[b];
 }
 
-// Stable scatter of one tile. The tile's 16 entries per lane are loaded up
-// front; 16 rounds then rank each entry among equal digits (ballots inside a
-// wave, double-buffered per-wave counts across waves and rounds) and place it
-// in LDS in digit order; finally consecutive lanes copy consecutive LDS slots
-// out, so each digit's run of the tile is written as one contiguous stretch.
+// Stable scatter of one tile. Wave w holds the tile's entries [w * 1024, (w + 1) * 1024),
+// 16 per lane, loaded up front (64 consecutive entries per round, coalesced). Each wave
+// ranks its entries among equal digits by itself: ballots over the digit bits give the
+// lanes of a round that share a digit, and a wave-private LDS count per digit carries the
+// rank across rounds (no block barrier per round). One barrier later, a per-digit prefix
+// over the waves places each wave's run after the earlier waves' runs of the same digit,
+// which keeps the scatter stable (entries of one digit stay in tile order). The tile is
+// then placed in LDS in digit order and consecutive lanes copy consecutive LDS slots out,
+// so each digit's run of the tile is written as one contiguous stretch.
 // LAST: write perm and rowind instead of the next pass's payload. VALS: the
 // entries' values travel with them (val_in: the CSR values in the first pass,
 // or NULL for all ones), ending in val_out (vals_t in the last pass).
@@ -204,10 +208,10 @@ radix_downsweep_kernel(const int32_t* __restrict__ keys_in, const uint64_t* __re
                        int32_t* __restrict__ perm, int32_t* __restrict__ rowind, const float* __restrict__ val_in,
                        float* __restrict__ val_out) {
   constexpr int kWaves = kRadixThreads / 64;
+  constexpr int kPerWave = kTransposeTile / kWaves;  // 1024 = kRadixItems rounds of 64
   __shared__ int32_t s_base[kRadixMaxBuckets];    // global start of each digit's run of this tile
   __shared__ int32_t s_lstart[kRadixMaxBuckets];  // tile-local start of each digit's run
-  __shared__ int32_t s_run[kRadixMaxBuckets];     // entries of each digit placed in earlier rounds
-  __shared__ int16_t s_wc[2][kWaves][kRadixMaxBuckets];
+  __shared__ int32_t s_wc[kWaves][kRadixMaxBuckets];  // per-wave digit counts, then their prefixes
   __shared__ int32_t s_key[kTransposeTile];
   __shared__ uint64_t s_pay[kTransposeTile];
   __shared__ float s_val[VALS ? kTransposeTile : 1];
@@ -216,11 +220,7 @@ radix_downsweep_kernel(const int32_t* __restrict__ keys_in, const uint64_t* __re
   const int wave = threadIdx.x >> 6;
   for (int b = threadIdx.x; b < buckets; b += kRadixThreads) {
     s_base[b] = offs[static_cast<int64_t>(b) * ntiles + tile];
-    s_run[b] = 0;
-    for (int w = 0; w < kWaves; ++w) {
-      s_wc[0][w][b] = 0;
-      s_wc[1][w][b] = 0;
-    }
+    for (int w = 0; w < kWaves; ++w) s_wc[w][b] = 0;
   }
   if (wave == 0) {  // exclusive scan of this tile's digit counts
     const int per = (buckets + 63) / 64;
@@ -241,23 +241,24 @@ radix_downsweep_kernel(const int32_t* __restrict__ keys_in, const uint64_t* __re
   const uint64_t below = (1ull << lane) - 1ull;
   const int64_t base = tile * kTransposeTile;
   const int tile_n = static_cast<int>(nnz - base < kTransposeTile ? nnz - base : kTransposeTile);
+  const int wbase = wave * kPerWave;
   int32_t key[kRadixItems];
   uint64_t pay[kRadixItems];
   float val[kRadixItems];
+  int32_t lpos[kRadixItems];
 #pragma unroll
   for (int j = 0; j < kRadixItems; ++j) {
-    const int i = j * kRadixThreads + threadIdx.x;
+    const int i = wbase + j * 64 + lane;
     key[j] = i < tile_n ? keys_in[base + i] : 0;
     pay[j] = i < tile_n ? pay_in[base + i] : 0ull;
     if constexpr (VALS) val[j] = (i < tile_n && val_in) ? val_in[base + i] : 1.0f;
   }
-  __syncthreads();
+  __syncthreads();  // s_wc zeroed, s_lstart and s_base written
 #pragma unroll
   for (int j = 0; j < kRadixItems; ++j) {
-    const int buf = j & 1;
-    const bool valid = j * kRadixThreads + static_cast<int>(threadIdx.x) < tile_n;
+    const bool valid = wbase + j * 64 + lane < tile_n;
     const int d = (key[j] >> shift) & mask;
-    // lanes of this wave holding the same digit
+    // lanes of this round holding the same digit
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < dbits; ++b) {
       const bool bit = (d >> b) & 1;
@@ -265,23 +266,29 @@ radix_downsweep_kernel(const int32_t* __restrict__ keys_in, const uint64_t* __re
       peers &= bit ? m : ~m;
     }
     const int rank_w = __popcll(peers & below);
-    if (valid && rank_w == 0) s_wc[buf][wave][d] = static_cast<int16_t>(__popcll(peers));
-    __syncthreads();  // this round's counts visible; the previous round's s_run update done
-    if (valid) {
-      int slot = s_lstart[d] + s_run[d] + rank_w;
-      for (int w = 0; w < wave; ++w) slot += s_wc[buf][w][d];
+    // the wave's own count of digit d so far (only this wave writes s_wc[wave])
+    const int c = valid ? s_wc[wave][d] : 0;
+    lpos[j] = c + rank_w;
+    if (valid && rank_w == 0) s_wc[wave][d] = c + __popcll(peers);
+  }
+  __syncthreads();  // every wave's counts final
+  for (int b = threadIdx.x; b < buckets; b += kRadixThreads) {
+    int run = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      const int cw = s_wc[w][b];
+      s_wc[w][b] = run;
+      run += cw;
+    }
+  }
+  __syncthreads();  // per-wave prefixes in s_wc
+#pragma unroll
+  for (int j = 0; j < kRadixItems; ++j) {
+    if (wbase + j * 64 + lane < tile_n) {
+      const int d = (key[j] >> shift) & mask;
+      const int slot = s_lstart[d] + s_wc[wave][d] + lpos[j];
       s_key[slot] = key[j];
       s_pay[slot] = pay[j];
       if constexpr (VALS) s_val[slot] = val[j];
-    }
-    __syncthreads();  // every lane has read s_run and this round's counts
-    for (int b = threadIdx.x; b < buckets; b += kRadixThreads) {
-      int add = 0;
-      for (int w = 0; w < kWaves; ++w) {
-        add += s_wc[buf][w][b];
-        s_wc[buf][w][b] = 0;
-      }
-      s_run[b] += add;
     }
   }
   __syncthreads();  // the tile sits in LDS in digit order
