@@ -59,6 +59,12 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   // so one 32 x 32 -> 64-bit multiply-add forms the address.
   const uint32_t ldb32 = static_cast<uint32_t>(p.ldb);
   auto row_at = [&](int64_t k) { return Bc + static_cast<uint64_t>(static_cast<uint32_t>(k)) * ldb32; };
+  // segment_coo streams its src rows once, in order: non-temporal loads (src 1.07 GB,
+  // N = 64 fp32: ~10 % faster); the SpMM gathers reuse B rows and keep plain loads
+  auto load_row = [&](const VT* q, RawCols<VT, CF>& r) {
+    if constexpr (SEG) load_raw_stream<VT, CF>(q, r);
+    else load_raw<VT, CF>(q, r);
+  };
   // Every lane of the wave holds CF columns: full tiles can skip per-lane column checks.
   const bool lanes_full = __ballot(!full) == 0;
   const int sbase = g * G;  // this group's LDS slots
@@ -225,10 +231,10 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
             } else {
               kk = *reinterpret_cast<const int4*>(s_k + sbase + pp + u);
             }
-            load_raw<VT, CF>(row_at(kk.x), raw[u]);
-            load_raw<VT, CF>(row_at(kk.y), raw[u + 1]);
-            load_raw<VT, CF>(row_at(kk.z), raw[u + 2]);
-            load_raw<VT, CF>(row_at(kk.w), raw[u + 3]);
+            load_row(row_at(kk.x), raw[u]);
+            load_row(row_at(kk.y), raw[u + 1]);
+            load_row(row_at(kk.z), raw[u + 2]);
+            load_row(row_at(kk.w), raw[u + 3]);
           }
           if (pp == 0) prefetch(off + G);
 #pragma unroll
@@ -260,7 +266,7 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
         for (int u = 0; u < U; ++u) {
           if (full && pp + u < cnt) {
             const int64_t krow = SEG ? (c0 + off + pp + u) : static_cast<int64_t>(s_k[sbase + pp + u]);
-            load_raw<VT, CF>(row_at(krow), raw[u]);
+            load_row(row_at(krow), raw[u]);
           }
         }
         // The next tile's metadata travels while this batch's rows arrive.
