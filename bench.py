@@ -83,7 +83,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-tuned", action="store_true", help="skip the comparison legs (for profiling)")
     ap.add_argument("--dump", type=str, default=None, help="write tuning results to this JSON file")
-    ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "fwd_bwd_csr_values", "values_permute", "segment_f32",
+    ap.add_argument("--only-leg", choices=["bf16", "fwd_bwd", "fwd_bwd_csr_values", "fwd_bwd_step", "values_permute",
+                                           "segment_f32",
                                            "segment_bf16", "banded_f32", "banded_bf16", "banded_f16",
                                            "batched_graph", "batched_graph_kernels"],
                     default=None,
@@ -160,7 +161,8 @@ def only_leg(args):
     """One leg of the bench alone, `warmup` + `steps` calls, for rocprofv3 summaries of a
     single kernel configuration: the tuned bf16-B SpMM, the fused backward (full: dB and
     dvals; values and dvals in the transpose's order, or for fwd_bwd_csr_values in A's
-    order through perm), segment_coo AUTO over the power-law index (E = 4.19M,
+    order through perm), the whole fp32 step (fwd_bwd_step: the permutation plan's first
+    pass, the tuned forward reading values through it, the fused backward), segment_coo AUTO over the power-law index (E = 4.19M,
     N = 64), or one replay of the batched step's hipGraph (256 uneven graphs, 2.39M nodes,
     9.58M nnz, bf16 features, N = 64: forward SpMM, per-graph sum pooling and the backward
     for dH = dP[graph]) as the batch plan's single launch (batched_graph) or kernel by
@@ -212,6 +214,26 @@ def only_leg(args):
         plan = ops.PermutePlan(iperm)
         vals_a = torch.empty_like(vals_t)
         fn = lambda: plan(vals_t, out=vals_a)  # noqa: E731
+    elif args.only_leg == "fwd_bwd_step":
+        # the whole fp32 training step of the fwd_bwd_ms line: values kept in the
+        # transpose's order, read by the tuned forward through the permutation plan, fused
+        # backward
+        dC = torch.from_numpy(rng.standard_normal((m.nrow, N)).astype(np.float32)).to(dev)
+        T = ops.csr_transpose(A)
+        e = autotune.load_table().get(autotune.matrix_key("spmm", torch.float32, N, stats, m.ncol))
+        v, k = (e["variant"], autotune.resolve_knobs(e.get("knobs"), stats)) if e else ("auto", None)
+        C = torch.empty((m.nrow, N), dtype=torch.float32, device=dev)
+        dB = torch.empty_like(B32)
+        dvals = torch.empty(m.nnz, dtype=torch.float32, device=dev)
+        ws_f = ops.spmm_workspace(A, N, torch.float32, v, k)
+        ws_b = ops.spmm_backward_workspace(T, N, torch.float32)
+        vals_t = A.vals.index_select(0, T.perm.long())
+        plan = ops.PermutePlan(ops.invert_permutation(T.perm))
+
+        def fn():
+            plan.spmm(A, vals_t, B32, variant=v, knobs=k, out=C, workspace=ws_f)
+            ops.spmm_backward(T, None, dC, B32, dB=dB, dvals=dvals, workspace=ws_b, vals_t=vals_t,
+                              dvals_at_order=True)
     elif args.only_leg in ("fwd_bwd", "fwd_bwd_csr_values"):
         dC = torch.from_numpy(rng.standard_normal((m.nrow, N)).astype(np.float32)).to(dev)
         T = ops.csr_transpose(A)
@@ -386,10 +408,11 @@ def main():
         ws_b = ops.spmm_backward_workspace(T, N, torch.float32)
 
         # learnable values kept in the transpose's order (the autograd layer's
-        # values_order="transpose"): each step permutes them into A's order with the two-pass
-        # plan and runs the forward on that copy; the backward reads them and writes dvals
-        # coalesced. fwd_bwd_vals_map reads them through the inverse permutation inside the
-        # forward SpMM instead.
+        # values_order="transpose"): each step runs the two-pass permutation plan's first
+        # pass and the forward SpMM reads every value from the plan's staged bucket (the
+        # second pass fused into it); the backward reads them and writes dvals coalesced.
+        # fwd_bwd_permute_then_spmm runs both passes into an A-order copy first;
+        # fwd_bwd_vals_map reads them through the inverse permutation inside the forward.
         iperm = ops.invert_permutation(T.perm)
         vals_t = A.vals.index_select(0, T.perm.long())
         A_at = A.with_vals_map(vals_t, iperm)
@@ -398,6 +421,13 @@ def main():
         A_perm = A.with_vals(vals_a)
 
         def fwd_bwd():
+            # the plan's first pass, then the SpMM reading each value from its staged bucket
+            perm_plan.spmm(A, vals_t, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
+            ops.spmm_backward(T, None, dC, B32, dB=dB, dvals=dvals, workspace=ws_b, vals_t=vals_t,
+                              dvals_at_order=True)
+
+        def fwd_bwd_permute_then_spmm():
+            # both passes into an A-order copy, then the SpMM on it
             perm_plan(vals_t, out=vals_a)
             ops.spmm(A_perm, B32, variant=v32, knobs=k32, out=C, workspace=ws_f)
             ops.spmm_backward(T, None, dC, B32, dB=dB, dvals=dvals, workspace=ws_b, vals_t=vals_t,
@@ -409,6 +439,12 @@ def main():
         ws_bb = ops.spmm_backward_workspace(T, N, torch.float32)
 
         def fwd_bwd_bf16():
+            perm_plan.spmm(A, vals_t, Bbf, variant=tuned[torch.bfloat16][0], knobs=tuned[torch.bfloat16][1], out=C,
+                           workspace=ws_fb)
+            ops.spmm_backward(T, None, dC, Bbf, dB=dB, dvals=dvals, workspace=ws_bb, vals_t=vals_t,
+                              dvals_at_order=True)
+
+        def fwd_bwd_bf16_permute_then_spmm():
             perm_plan(vals_t, out=vals_a)
             ops.spmm(A_perm, Bbf, variant=tuned[torch.bfloat16][0], knobs=tuned[torch.bfloat16][1], out=C,
                      workspace=ws_fb)
@@ -466,7 +502,11 @@ def main():
             Tn = ops.csr_transpose(A)
             ops.spmm_backward(Tn, A.vals, dC, B32, dB=dB, dvals=dvals, workspace=ws_b)
 
-        fb_ms, _ = timed(fwd_bwd, 10, 3, kernel_events=False)
+        # 30 steps back to back per leg: the host's latency to its first launch, paid once
+        # per timed region, stays under ~1 % of a step
+        # 30 steps back to back per leg: the host's latency to the first launch of a timed
+        # region is paid once per region and stays under ~1 % of a step
+        fb_ms, _ = timed(fwd_bwd, 30, 5, kernel_events=False)
         # the same step captured as one hipGraph (launch gaps between its six kernels shrink)
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
@@ -477,15 +517,17 @@ def main():
         step_graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(step_graph):
             fwd_bwd()
-        fbg_ms, _ = timed(step_graph.replay, 10, 3, kernel_events=False)
-        fbm_ms, _ = timed(fwd_bwd_vals_map, 10, 3, kernel_events=False)
-        fbb_ms, _ = timed(fwd_bwd_bf16, 10, 3, kernel_events=False)
+        fbg_ms, _ = timed(step_graph.replay, 30, 5, kernel_events=False)
+        fbm_ms, _ = timed(fwd_bwd_vals_map, 30, 5, kernel_events=False)
+        fbu_ms, _ = timed(fwd_bwd_permute_then_spmm, 30, 5, kernel_events=False)
+        fbb_ms, _ = timed(fwd_bwd_bf16, 30, 5, kernel_events=False)
+        fbbu_ms, _ = timed(fwd_bwd_bf16_permute_then_spmm, 30, 5, kernel_events=False)
         fbp_ms, _ = timed(fwd_bwd_plain, 5, 2, kernel_events=False)
         perm_ms, _ = timed(permute_only, 20, 3, kernel_events=False)
-        fbc_ms, _ = timed(fwd_bwd_csr_values, 10, 3, kernel_events=False)
-        fbw_ms, _ = timed(fwd_bwd_csr_values_in_walk, 10, 3, kernel_events=False)
-        fbs_ms, _ = timed(fwd_bwd_separate, 10, 3, kernel_events=False)
-        fbf_ms, _ = timed(fwd_bwd_fixed_vals, 10, 3, kernel_events=False)
+        fbc_ms, _ = timed(fwd_bwd_csr_values, 30, 5, kernel_events=False)
+        fbw_ms, _ = timed(fwd_bwd_csr_values_in_walk, 30, 5, kernel_events=False)
+        fbs_ms, _ = timed(fwd_bwd_separate, 30, 5, kernel_events=False)
+        fbf_ms, _ = timed(fwd_bwd_fixed_vals, 30, 5, kernel_events=False)
         fbt_ms, _ = timed(fwd_bwd_with_transpose, 5, 2, kernel_events=False)
         plain_ms = plain["plain_ebpr"]["ms"]
         # hipSPARSE on the same inputs (the reference's cuSPARSE comparison, README §5)
@@ -516,8 +558,10 @@ def main():
             "hipsparse_best_over_tuned_f32": None if best_vendor is None else round(best_vendor / ms_step, 3),
             "fwd_bwd_ms": round(fb_ms, 4),
             "fwd_bwd_ms_vals_map": round(fbm_ms, 4),
+            "fwd_bwd_ms_permute_then_spmm": round(fbu_ms, 4),
             "fwd_bwd_ms_hipgraph": round(fbg_ms, 4),
             "fwd_bwd_ms_bf16": round(fbb_ms, 4),
+            "fwd_bwd_ms_bf16_permute_then_spmm": round(fbbu_ms, 4),
             "fwd_bwd_ms_plain_f32": round(fbp_ms, 4),
             "fwd_bwd_speedup_bf16_vs_plain_f32": round(fbp_ms / fbb_ms, 2),
             "values_permute_ms": round(perm_ms, 4),

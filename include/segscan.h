@@ -301,7 +301,8 @@ segscan_status_t segscan_invert_permutation(const int32_t* perm, int64_t n, int3
  * transpose's order, made into A's order once per step (src = segscan_invert_permutation(perm))
  * so the forward SpMM reads them coalesced. Creation reads src back, checks it is a
  * permutation (SEGSCAN_EINVAL otherwise), builds the plan's tables on the host and uploads
- * them (8 bytes per value + a 4-byte-per-value staging buffer, allocated here); it
+ * them (8 bytes per value + two 4-byte-per-value buffers, staging and an A-order copy for
+ * segscan_spmm_csr_permuted, allocated here); it
  * synchronises `stream`. Runs of one plan must not overlap (they share the staging buffer);
  * in and out must not alias. No counterpart in the reference (it has no backward). */
 typedef struct segscan_permute_plan* segscan_permute_plan_t;
@@ -309,6 +310,23 @@ segscan_status_t segscan_permute_plan_create(const int32_t* src, int64_t n, void
                                              segscan_permute_plan_t* plan);
 segscan_status_t segscan_permute_run(segscan_permute_plan_t plan, const float* in, float* out, void* stream);
 segscan_status_t segscan_permute_plan_destroy(segscan_permute_plan_t plan);
+/* SpMM whose values arrive through a permutation plan: C (=|+=) A'.B where A' is A's
+ * pattern with value p = vals_src[src[p]] (src the plan's permutation, n = A.nnz; A.vals and
+ * A.vals_map must be NULL). Equals segscan_permute_run into a temporary followed by
+ * segscan_spmm_csr on it, bit for bit, with the same variant, knobs, flags and workspace.
+ * When the resolved launch is ebsr and each workgroup round covers a part of one of the
+ * plan's 8192-value buckets (thread_nz x group size x workgroups' groups dividing 8192, at
+ * least 2048), the plan's second pass is fused into the SpMM: each workgroup loads its
+ * bucket of staged values into LDS and reads them through 16-bit slots, so the A-order copy
+ * is never written or read. Other launches run both passes into a buffer the plan holds.
+ * Calls on one plan must not overlap. No counterpart in the reference (its values are
+ * never learned). */
+segscan_status_t segscan_spmm_csr_permuted(const segscan_csr_t* A, segscan_permute_plan_t plan,
+                                           const float* vals_src, int64_t N, const void* B, int64_t ldb,
+                                           segscan_dtype_t b_dtype, float* C, int64_t ldc,
+                                           segscan_spmm_variant_t variant, const segscan_knobs_t* knobs,
+                                           uint32_t flags, void* workspace, size_t workspace_bytes,
+                                           void* stream);
 /* counts[s] = #{e : index[e] == s}, s < dst_len (sorted index). */
 segscan_status_t segscan_segment_counts(const int32_t* index, int64_t E, int64_t dst_len, int32_t* counts,
                                         void* stream);

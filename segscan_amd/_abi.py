@@ -143,6 +143,9 @@ _SIGNATURES = {
     "segscan_invert_permutation": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
     "segscan_permute_plan_create": (c_int32, [c_void_p, c_int64, c_void_p, POINTER(c_void_p)]),
     "segscan_permute_run": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "segscan_spmm_csr_permuted": (c_int32, [POINTER(CsrDesc), c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int32,
+                                            c_void_p, c_int64, c_int32, POINTER(Knobs), c_uint32, c_void_p, c_size_t,
+                                            c_void_p]),
     "segscan_permute_plan_destroy": (c_int32, [c_void_p]),
     "segscan_spmm_csr_noalloc": (c_int32, [POINTER(CsrDesc), c_int64, c_void_p, c_int64, c_int32, c_void_p, c_int64,
                                            c_int32, POINTER(Knobs), c_uint32, c_void_p]),

@@ -4,8 +4,9 @@ CPU.
 
 SpMM  C = A(vals) . B, A fixed pattern (rowptr, colind), vals and B learnable
 (vals in CSR order, or with values_order="transpose" in the transpose's order: the
-forward takes them into A's order with a two-pass permutation plan (or, with
-forward_values="vals_map", reads them through the inverse permutation inside the SpMM)
+forward reads them through a two-pass permutation plan, whose second pass the SpMM takes
+over where its launch allows (segscan_spmm_csr_permuted), or, with
+forward_values="vals_map", through the inverse permutation inside the SpMM)
 and the fused backward reads them and writes their gradient coalesced, with no random
 access per entry; for CSR-order values the backward permutes them into the transpose's
 order and the gradient back with two such plans, or with backward_values="perm" reads
@@ -116,8 +117,15 @@ class SpmmPattern:
 class SpmmFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vals: torch.Tensor, B: torch.Tensor, pattern: SpmmPattern):
-        A = pattern.with_vals(vals.detach().float().contiguous())
-        C = ops.spmm(A, B.detach().contiguous(), variant=pattern.variant, knobs=pattern.knobs)
+        v32 = vals.detach().float().contiguous()
+        if pattern.permute is not None:
+            # values kept in the transpose's order, read through the plan (its second pass
+            # fused into the SpMM where the launch allows)
+            C = pattern.permute.spmm(pattern.A, v32, B.detach().contiguous(), variant=pattern.variant,
+                                     knobs=pattern.knobs)
+        else:
+            C = ops.spmm(pattern.with_vals(v32), B.detach().contiguous(), variant=pattern.variant,
+                         knobs=pattern.knobs)
         ctx.save_for_backward(vals, B)
         ctx.pattern = pattern
         return C

@@ -16,6 +16,11 @@ enum class VType : int { F32 = 0, BF16 = 1, F16 = 2 };
 // robin: workgroup w runs on XCD w % kXcds.
 constexpr int kXcds = 8;
 
+// Permutation plans (permute.hip) stage values in buckets of kPermBucket consecutive
+// output positions; an ebsr SpMM whose workgroups each cover a part of one bucket reads
+// its values from that bucket (SrParams::vslot).
+constexpr int kPermBucket = 8192;
+
 // Owner write policy: empty output rows are tracked in blocks of kZeroRows rows, one
 // flag byte per block (owner_empty_run, carry_fixup_kernel).
 constexpr int kZeroRowsLog = 4;
@@ -58,6 +63,11 @@ struct SrParams {
   const int32_t* colind;
   const float* vals;
   const int32_t* vmap;        // nullable: entry e's value is vals[vmap[e]]
+  // nullable (SpMM): values staged by a permutation plan's first pass. Entry e's value is
+  // vmid[(e / kPermBucket) * kPermBucket + vslot[e]]; chunk x groups per workgroup divides
+  // kPermBucket, so each workgroup round reads one bucket, loaded into LDS.
+  const float* vmid;
+  const uint16_t* vslot;
   const int32_t* index;
   const void* B;
   int64_t ldb;

@@ -20,12 +20,13 @@
 
 #include "../../include/segscan.h"
 #include "abi_internal.hpp"
+#include "kernel_params.hpp"
 
 namespace segscan {
 namespace {
 
 constexpr int kTile = 16384;    // values per pass-1 tile (64 KB of LDS)
-constexpr int kBucket = 8192;   // outputs per pass-2 bucket (32 KB of LDS)
+constexpr int kBucket = kPermBucket;  // outputs per pass-2 bucket (8192: 32 KB of LDS)
 constexpr int kPermTpb = 1024;
 
 __global__ void __launch_bounds__(kPermTpb)
@@ -56,12 +57,38 @@ permute_pass2_kernel(const float* __restrict__ mid, const uint16_t* __restrict__
 struct segscan_permute_plan {
   int device = 0;
   int64_t n = 0;
-  void* mem = nullptr;  // rank1, slot2 (uint16), mid_pos (int32), mid (float)
+  void* mem = nullptr;  // rank1, slot2 (uint16), mid_pos (int32), mid, aux (float)
   uint16_t* rank1 = nullptr;
   uint16_t* slot2 = nullptr;
   int32_t* mid_pos = nullptr;
   float* mid = nullptr;
+  float* aux = nullptr;  // A-order values for segscan_spmm_csr_permuted's unfused launches
 };
+
+namespace segscan {
+
+PermuteBuffers permute_buffers(const segscan_permute_plan* plan) {
+  PermuteBuffers b;
+  b.n = plan->n;
+  b.mid = plan->mid;
+  b.slot2 = plan->slot2;
+  b.aux = plan->aux;
+  return b;
+}
+
+hipError_t permute_pass1(const segscan_permute_plan* plan, const float* in, hipStream_t s) {
+  const unsigned tiles = static_cast<unsigned>((plan->n + kTile - 1) / kTile);
+  permute_pass1_kernel<<<tiles, kPermTpb, 0, s>>>(in, plan->rank1, plan->mid_pos, plan->mid, plan->n);
+  return hipGetLastError();
+}
+
+hipError_t permute_pass2(const segscan_permute_plan* plan, float* out, hipStream_t s) {
+  const unsigned buckets = static_cast<unsigned>((plan->n + kBucket - 1) / kBucket);
+  permute_pass2_kernel<<<buckets, kPermTpb, 0, s>>>(plan->mid, plan->slot2, out, plan->n);
+  return hipGetLastError();
+}
+
+}  // namespace segscan
 
 namespace {
 
@@ -131,7 +158,7 @@ segscan_status_t segscan_permute_plan_create(const int32_t* src, int64_t n, void
   pl->device = device;
   pl->n = n;
   const size_t b16 = a256(static_cast<size_t>(n) * 2), b32 = a256(static_cast<size_t>(n) * 4);
-  e = hipMalloc(&pl->mem, 2 * b16 + 2 * b32);
+  e = hipMalloc(&pl->mem, 2 * b16 + 3 * b32);
   if (e != hipSuccess) {
     delete pl;
     return hip_fail(e, "permute plan: allocation");
@@ -141,6 +168,7 @@ segscan_status_t segscan_permute_plan_create(const int32_t* src, int64_t n, void
   pl->slot2 = reinterpret_cast<uint16_t*>(base + b16);
   pl->mid_pos = reinterpret_cast<int32_t*>(base + 2 * b16);
   pl->mid = reinterpret_cast<float*>(base + 2 * b16 + b32);
+  pl->aux = reinterpret_cast<float*>(base + 2 * b16 + 2 * b32);
   e = hipMemcpyAsync(pl->rank1, rank1.data(), static_cast<size_t>(n) * 2, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(pl->slot2, slot2.data(), static_cast<size_t>(n) * 2, hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
@@ -161,15 +189,8 @@ segscan_status_t segscan_permute_run(segscan_permute_plan_t plan, const float* i
   if (!in || !out) return einval("in or out is NULL");
   if (in == out) return einval("in and out must not alias");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int64_t n = plan->n;
-  const unsigned tiles = static_cast<unsigned>((n + segscan::kTile - 1) / segscan::kTile);
-  const unsigned buckets = static_cast<unsigned>((n + segscan::kBucket - 1) / segscan::kBucket);
-  segscan::permute_pass1_kernel<<<tiles, segscan::kPermTpb, 0, s>>>(in, plan->rank1, plan->mid_pos, plan->mid, n);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) {
-    segscan::permute_pass2_kernel<<<buckets, segscan::kPermTpb, 0, s>>>(plan->mid, plan->slot2, out, n);
-    e = hipGetLastError();
-  }
+  hipError_t e = segscan::permute_pass1(plan, in, s);
+  if (e == hipSuccess) e = segscan::permute_pass2(plan, out, s);
   if (e != hipSuccess) return hip_fail(e, "permute launch");
   segscan::set_error(SEGSCAN_OK, "");
   return SEGSCAN_OK;

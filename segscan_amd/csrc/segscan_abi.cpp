@@ -701,6 +701,49 @@ segscan_status_t segscan_spmm_csr(const segscan_csr_t* A, int64_t N, const void*
                                 stream, nullptr);
 }
 
+segscan_status_t segscan_spmm_csr_permuted(const segscan_csr_t* A, segscan_permute_plan_t plan,
+                                           const float* vals_src, int64_t N, const void* B, int64_t ldb,
+                                           segscan_dtype_t b_dtype, float* C, int64_t ldc,
+                                           segscan_spmm_variant_t variant, const segscan_knobs_t* knobs,
+                                           uint32_t flags, void* workspace, size_t workspace_bytes,
+                                           void* stream) {
+  using segscan::spmm_csr_impl;
+  if (!A) return fail(SEGSCAN_EINVAL, "A is NULL");
+  if (!plan) return fail(SEGSCAN_EINVAL, "plan is NULL");
+  if (A->vals || A->vals_map)
+    return fail(SEGSCAN_EINVAL, "A.vals and A.vals_map must be NULL: the values come from vals_src through the plan");
+  const segscan::PermuteBuffers pb = segscan::permute_buffers(plan);
+  if (pb.n != A->nnz)
+    return fail(SEGSCAN_EINVAL, "plan permutes %lld values, A has %lld entries", (long long)pb.n, (long long)A->nnz);
+  if (!vals_src) return fail(SEGSCAN_EINVAL, "vals_src is NULL");
+  if (reinterpret_cast<uintptr_t>(vals_src) % 4) return fail(SEGSCAN_EINVAL, "vals_src is not 4-byte aligned");
+  // validation and planning first: nothing is launched for a call that would fail
+  segscan::StagedValues sv;
+  sv.mid = pb.mid;
+  sv.slot = pb.slot2;
+  size_t need = 0;
+  segscan_status_t st = spmm_csr_impl(A, N, B, ldb, b_dtype, C, ldc, variant, knobs, flags, workspace,
+                                      workspace_bytes, stream, &need, &sv);
+  if (st) return st;
+  if (A->nrow == 0 || N == 0)
+    return spmm_csr_impl(A, N, B, ldb, b_dtype, C, ldc, variant, knobs, flags, workspace, workspace_bytes, stream,
+                         nullptr);
+  if (need > 0 && (!workspace || workspace_bytes < need))
+    return fail(SEGSCAN_EWORKSPACE, "workspace of %zu bytes required", need);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = segscan::permute_pass1(plan, vals_src, s);
+  if (e != hipSuccess) return hip_status(e, "permute pass 1");
+  if (sv.fused)
+    return spmm_csr_impl(A, N, B, ldb, b_dtype, C, ldc, variant, knobs, flags, workspace, workspace_bytes, stream,
+                         nullptr, &sv);
+  e = segscan::permute_pass2(plan, pb.aux, s);
+  if (e != hipSuccess) return hip_status(e, "permute pass 2");
+  segscan_csr_t a = *A;
+  a.vals = pb.aux;
+  return spmm_csr_impl(&a, N, B, ldb, b_dtype, C, ldc, variant, knobs, flags, workspace, workspace_bytes, stream,
+                       nullptr);
+}
+
 }  // extern "C"
 
 namespace segscan {
@@ -711,7 +754,7 @@ namespace segscan {
 segscan_status_t spmm_csr_impl(const segscan_csr_t* A, int64_t N, const void* B, int64_t ldb,
                                segscan_dtype_t b_dtype, float* C, int64_t ldc, segscan_spmm_variant_t variant,
                                const segscan_knobs_t* knobs, uint32_t flags, void* workspace, size_t workspace_bytes,
-                               void* stream, size_t* plan_bytes) {
+                               void* stream, size_t* plan_bytes, StagedValues* staged) {
   segscan_status_t st = validate_csr(A, true);
   if (st) return st;
   if (N < 0 || N > kMaxI32) return fail(SEGSCAN_EINVAL, "N = %lld out of range", (long long)N);
@@ -734,6 +777,13 @@ segscan_status_t spmm_csr_impl(const segscan_csr_t* A, int64_t N, const void* B,
   SpmmPlan pl;
   st = plan_spmm(M, nnz, A->max_row_nnz, std::max<int64_t>(N, 1), b_dtype, variant, from_c(knobs), cap, pl);
   if (st) return st;
+  if (staged) {
+    // a workgroup round spans chunk x groups entries from a multiple of that span
+    const int64_t span = pl.chunk * (pl.tpb / pl.G);
+    staged->fused = pl.fam == kEbsr && span >= 2048 && kPermBucket % span == 0;
+    if (!plan_bytes && !staged->fused)
+      return fail(SEGSCAN_EUNSUPPORTED, "staged values need an ebsr launch whose rounds cover part of one bucket");
+  }
   if (plan_bytes) {
     *plan_bytes = (M == 0 || N == 0 || nnz == 0) ? 0 : pl.ws_bytes;
     return ok();
@@ -820,6 +870,11 @@ segscan_status_t spmm_csr_impl(const segscan_csr_t* A, int64_t N, const void* B,
       }
       // chunk start rows are found by the kernel itself (no partition pass)
       l.shmem = static_cast<size_t>(pl.tpb) * 12;
+      if (staged) {
+        p.vmid = staged->mid;
+        p.vslot = staged->slot;
+        l.shmem += static_cast<size_t>(kPermBucket) * 4;
+      }
       mark_start(s);
       e = launch_spmm_sr(vt, pl.CF, pl.G, p, l, compiled);
       mark_stop(s);

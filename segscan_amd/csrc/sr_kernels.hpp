@@ -38,6 +38,7 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   int32_t* s_k = sr_smem;                                  // [tpb] column (SpMM)
   float* s_v = reinterpret_cast<float*>(sr_smem + p.tpb);  // [tpb] value (SpMM)
   int32_t* s_r = sr_smem + 2 * p.tpb;                      // [tpb] row / segment
+  float* s_bkt = reinterpret_cast<float*>(sr_smem + 3 * p.tpb);  // [kPermBucket] staged values (vslot)
 
   constexpr int U = (G < 16) ? G : 16;  // dense-row loads issued back to back per batch
   // Full tiles take the predicate-free loop when batches split into fours. (With
@@ -74,10 +75,26 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   // (segments flag entry-free blocks; for a CSR matrix the fix-up pass reads rowptr)
   const bool zrun = owner && !acc_mode;
   const bool zflags = SEG && zrun && blockIdx.y == 0;
+  // Values from a permutation plan's staged buckets: each workgroup round covers a part of
+  // one bucket (checked by the host), which the workgroup loads into LDS; rounds are then
+  // workgroup-uniform, and waves without a chunk in a round only join its barriers.
+  const bool vbucket = !SEG && p.vslot != nullptr;
 
   for (int64_t c = ggid;; c += p.total_groups) {
-    // Chunks grow with the lane index; the wave's first lane has the lowest one.
-    if (__ballot(c < p.nchunks) == 0) break;
+    if (vbucket) {
+      const int64_t cw = c - g;  // the workgroup's first chunk this round
+      if (cw >= p.nchunks) break;
+      __syncthreads();  // every lane is done with the previous round's bucket
+      const int64_t b0 = (cw * p.chunk / kPermBucket) * kPermBucket;
+      const int blen = static_cast<int>(min<int64_t>(kPermBucket, p.nnz - b0));
+      const float4* src4 = reinterpret_cast<const float4*>(p.vmid + b0);
+      for (int i = L; i < (blen >> 2); i += p.tpb) reinterpret_cast<float4*>(s_bkt)[i] = src4[i];
+      for (int i = (blen & ~3) + L; i < blen; i += p.tpb) s_bkt[i] = p.vmid[b0 + i];
+      // the barrier that publishes the bucket follows the chunk start search
+    } else if (__ballot(c < p.nchunks) == 0) {
+      // Chunks grow with the lane index; the wave's first lane has the lowest one.
+      break;
+    }
     const bool chunk_valid = c < p.nchunks;
     const int64_t c0 = c * p.chunk;
     const int64_t clen = chunk_valid ? min(p.chunk, p.nnz - c0) : 0;
@@ -136,7 +153,7 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
     // lane's nnz, and the rowptr window starting after r_hint (a row no later
     // than any nnz still to come in the chunk).
     int r_hint = first_row;
-    int pk = 0, prow = INT32_MAX, prp = INT32_MAX;
+    int pk = 0, prow = INT32_MAX, prp = INT32_MAX, pslot = 0;
     float pv = 0.0f;
     auto prefetch = [&](int64_t off_n) {
       const bool vn = off_n + g_lane < clen;
@@ -145,7 +162,8 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
         prow = vn ? p.index[en] : INT32_MAX;
       } else {
         pk = vn ? p.colind[en] : 0;
-        pv = vn ? (p.vals ? p.vals[p.vmap ? p.vmap[en] : en] : 1.0f) : 0.0f;
+        if (vbucket) pslot = vn ? p.vslot[en] : 0;  // read from the bucket when staged
+        else pv = vn ? (p.vals ? p.vals[p.vmap ? p.vmap[en] : en] : 1.0f) : 0.0f;
         if (off_n < clen) {
           const int jj = r_hint + 1 + g_lane;
           prp = (jj <= p.M) ? p.rowptr[jj] : INT32_MAX;
@@ -153,6 +171,9 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
       }
     };
     prefetch(0);
+    // the round's bucket is in LDS before the first tile is staged (the first tile's
+    // metadata loads are already in flight)
+    if (vbucket) __syncthreads();
 
     // Entry (row r, value v) whose dense row is in b: a new row flushes the previous one.
     auto consume = [&](int r, float v, const float (&b)[CF]) {
@@ -200,11 +221,12 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
         row = prow;
         if (zflags) flag_tile(row, valid, cnt);
       } else {
+        const float v = vbucket ? s_bkt[pslot] : pv;  // read ahead of the row search
         row = group_row_find<G>(p.rowptr, p.M, r_hint, prp, static_cast<int>(c0 + off + g_lane), valid);
         const int last_row = __shfl(row, wave_gbase + max(cnt, 1) - 1);
         if (cnt > 0) r_hint = last_row;
         s_k[L] = pk;
-        s_v[L] = pv;
+        s_v[L] = v;
       }
       s_r[L] = valid ? row : INT32_MAX;
       // The triples are read back only by lanes of the same wave: order the LDS

@@ -335,6 +335,45 @@ class PermutePlan:
         _abi.check(_abi.load().segscan_permute_run(self._h, _ptr(x), _ptr(out), _stream(stream)), "permute_run")
         return out
 
+    def spmm(self, A: "CsrMatrix", vals_src: torch.Tensor, B: torch.Tensor, variant="auto", knobs=None,
+             out: Optional[torch.Tensor] = None, accumulate: bool = False,
+             workspace: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+        """C (=|+=) A'.B with A' = A's pattern holding value p = vals_src[src[p]]
+        (segscan_spmm_csr_permuted): equal bit for bit to spmm(A.with_vals(self(vals_src)), B, ...).
+        An ebsr launch whose workgroup rounds cover part of one 8192-value bucket reads the
+        values from the plan's staging buffer (the second pass fused into the SpMM); other
+        launches run both passes. A's own values are ignored. The workspace is spmm's for the
+        same A, N, dtype, variant and knobs."""
+        _require_device(A.rowptr, B, vals_src)
+        if A.nnz != self.n:
+            raise ValueError(f"plan permutes {self.n} values, A has {A.nnz} entries")
+        if vals_src.dtype != torch.float32 or vals_src.numel() != self.n or not vals_src.is_contiguous():
+            raise ValueError(f"vals_src must be contiguous float32 with {self.n} elements")
+        lib = _abi.load()
+        N = B.size(1)
+        if B.size(0) != A.ncol:
+            raise ValueError(f"B has {B.size(0)} rows, A has {A.ncol} columns")
+        if out is None:
+            out = torch.empty((A.nrow, N), dtype=torch.float32, device=B.device)
+        if out.dtype != torch.float32 or tuple(out.shape) != (A.nrow, N):
+            raise ValueError("out must be float32 of shape (nrow, N)")
+        v = _abi.SPMM_VARIANTS[variant] if isinstance(variant, str) else int(variant)
+        kp = _abi.knobs_ptr(knobs)
+        desc = A.desc()
+        desc.vals = None
+        desc.vals_map = None
+        if workspace is None:
+            ws_bytes = c_size_t(0)
+            _abi.check(lib.segscan_spmm_workspace_size(byref(desc), N, _dtype_code(B), v, kp, byref(ws_bytes)),
+                       "spmm_workspace_size")
+            workspace = torch.empty(max(1, ws_bytes.value), dtype=torch.uint8, device=B.device)
+        st = lib.segscan_spmm_csr_permuted(byref(desc), self._h, _ptr(vals_src), N, _ptr(B), _leading_dim(B, "B"),
+                                           _dtype_code(B), _ptr(out), _leading_dim(out, "C"), v, kp,
+                                           _abi.ACCUMULATE if accumulate else 0, _ptr(workspace), workspace.numel(),
+                                           _stream(stream))
+        _abi.check(st, "spmm_csr_permuted")
+        return out
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
