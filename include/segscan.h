@@ -316,7 +316,7 @@ segscan_status_t segscan_permute_plan_destroy(segscan_permute_plan_t plan);
  * segscan_spmm_csr on it, bit for bit, with the same variant, knobs, flags and workspace.
  * When the resolved launch is ebsr and each workgroup round covers a part of one of the
  * plan's 8192-value buckets (thread_nz x group size x workgroups' groups dividing 8192, at
- * least 2048), the plan's second pass is fused into the SpMM: each workgroup loads its
+ * least 2048; groups of 8 to 64 lanes), the plan's second pass is fused into the SpMM: each workgroup loads its
  * bucket of staged values into LDS and reads them through 16-bit slots, so the A-order copy
  * is never written or read. Other launches run both passes into a buffer the plan holds.
  * Calls on one plan must not overlap. No counterpart in the reference (its values are

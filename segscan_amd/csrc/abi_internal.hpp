@@ -15,7 +15,8 @@ namespace segscan {
 // HIP call) and stores the workspace the launch needs (0 if it launches nothing).
 // Values an ebsr SpMM reads from a permutation plan's staged buckets instead of A.vals.
 // The planning run (plan_bytes non-NULL) sets `fused` when the resolved launch can read
-// them: ebsr, with each workgroup round covering a part of one bucket. A launch with a
+// them: ebsr with groups of 8 to 64 lanes, each workgroup round covering a part of one
+// bucket (at least 2048 entries). A launch with a
 // StagedValues whose `fused` is false is refused.
 struct StagedValues {
   const float* mid = nullptr;       // bucket-major staging buffer, kPermBucket values a bucket

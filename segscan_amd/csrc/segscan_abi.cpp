@@ -778,9 +778,10 @@ segscan_status_t spmm_csr_impl(const segscan_csr_t* A, int64_t N, const void* B,
   st = plan_spmm(M, nnz, A->max_row_nnz, std::max<int64_t>(N, 1), b_dtype, variant, from_c(knobs), cap, pl);
   if (st) return st;
   if (staged) {
-    // a workgroup round spans chunk x groups entries from a multiple of that span
+    // a workgroup round spans chunk x groups entries from a multiple of that span; the
+    // staged-value kernels are built for groups of 8 to 64 lanes
     const int64_t span = pl.chunk * (pl.tpb / pl.G);
-    staged->fused = pl.fam == kEbsr && span >= 2048 && kPermBucket % span == 0;
+    staged->fused = pl.fam == kEbsr && pl.G >= 8 && span >= 2048 && kPermBucket % span == 0;
     if (!plan_bytes && !staged->fused)
       return fail(SEGSCAN_EUNSUPPORTED, "staged values need an ebsr launch whose rounds cover part of one bucket");
   }

@@ -32,8 +32,10 @@
 
 namespace segscan {
 
-template <typename VT, int CF, int G, bool SEG>
+// VB: values from a permutation plan's staged buckets (SrParams::vslot); SpMM only.
+template <typename VT, int CF, int G, bool SEG, bool VB = false>
 __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
+  static_assert(!(SEG && VB), "staged values are an SpMM mode");
   extern __shared__ int32_t sr_smem[];
   int32_t* s_k = sr_smem;                                  // [tpb] column (SpMM)
   float* s_v = reinterpret_cast<float*>(sr_smem + p.tpb);  // [tpb] value (SpMM)
@@ -78,7 +80,7 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   // Values from a permutation plan's staged buckets: each workgroup round covers a part of
   // one bucket (checked by the host), which the workgroup loads into LDS; rounds are then
   // workgroup-uniform, and waves without a chunk in a round only join its barriers.
-  const bool vbucket = !SEG && p.vslot != nullptr;
+  constexpr bool vbucket = VB;
 
   for (int64_t c = ggid;; c += p.total_groups) {
     if (vbucket) {
@@ -457,10 +459,22 @@ __global__ void __launch_bounds__(256) sddmm_sr_kernel(SddmmParams p) {
 
 // ---- instantiation and dispatch ----------------------------------------------------------
 
+// Staged-value launches (p.vslot set) exist for groups of 8 to 64 lanes; the host plans
+// them only there.
 template <typename VT, int CF, int G>
-static hipError_t launch_sr_t(const SrParams& p, const Launch& l, bool seg) {
-  if (seg) sr_kernel<VT, CF, G, true><<<l.grid, l.block, l.shmem, l.stream>>>(p);
-  else sr_kernel<VT, CF, G, false><<<l.grid, l.block, l.shmem, l.stream>>>(p);
+static hipError_t launch_sr_t(const SrParams& p, const Launch& l, bool seg, bool& ok) {
+  if (seg) {
+    sr_kernel<VT, CF, G, true><<<l.grid, l.block, l.shmem, l.stream>>>(p);
+  } else if (p.vslot) {
+    if constexpr (G >= 8) {
+      sr_kernel<VT, CF, G, false, true><<<l.grid, l.block, l.shmem, l.stream>>>(p);
+    } else {
+      ok = false;
+      return hipSuccess;
+    }
+  } else {
+    sr_kernel<VT, CF, G, false><<<l.grid, l.block, l.shmem, l.stream>>>(p);
+  }
   return hipGetLastError();
 }
 
@@ -468,13 +482,13 @@ template <typename VT, int CF>
 static hipError_t launch_sr_g(int g, const SrParams& p, const Launch& l, bool seg, bool& ok) {
   ok = true;
   switch (g) {
-    case 1: return launch_sr_t<VT, CF, 1>(p, l, seg);
-    case 2: return launch_sr_t<VT, CF, 2>(p, l, seg);
-    case 4: return launch_sr_t<VT, CF, 4>(p, l, seg);
-    case 8: return launch_sr_t<VT, CF, 8>(p, l, seg);
-    case 16: return launch_sr_t<VT, CF, 16>(p, l, seg);
-    case 32: return launch_sr_t<VT, CF, 32>(p, l, seg);
-    case 64: return launch_sr_t<VT, CF, 64>(p, l, seg);
+    case 1: return launch_sr_t<VT, CF, 1>(p, l, seg, ok);
+    case 2: return launch_sr_t<VT, CF, 2>(p, l, seg, ok);
+    case 4: return launch_sr_t<VT, CF, 4>(p, l, seg, ok);
+    case 8: return launch_sr_t<VT, CF, 8>(p, l, seg, ok);
+    case 16: return launch_sr_t<VT, CF, 16>(p, l, seg, ok);
+    case 32: return launch_sr_t<VT, CF, 32>(p, l, seg, ok);
+    case 64: return launch_sr_t<VT, CF, 64>(p, l, seg, ok);
     default: ok = false; return hipSuccess;
   }
 }

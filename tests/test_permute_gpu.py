@@ -150,15 +150,18 @@ _FUSED = [
 _UNFUSED = [
     ("ebsr", dict(group_factor=5, thread_nz=4, thread_per_block=256, coarsen_factor=2)),  # 1024-entry rounds
     ("ebsr", dict(group_factor=5, thread_nz=32, thread_per_block=192, coarsen_factor=2)),  # 6144: not a divisor
+    ("ebsr", dict(group_factor=2, thread_nz=32, thread_per_block=64, coarsen_factor=8)),  # 2048, 4-lane groups
     ("rbsr", None),
     ("ebpr", None),
 ]
 
 
 def _fuses(info) -> bool:
-    """The library's rule: ebsr, with a workgroup round's entries dividing 8192, >= 2048."""
+    """The library's rule: ebsr with groups of >= 8 lanes, a workgroup round's entries
+    dividing 8192, >= 2048."""
     span = info["chunk"] * (info["thread_per_block"] // info["group_size"])
-    return info["variant"] == _abi.SPMM_EBSR and span >= 2048 and 8192 % span == 0
+    return (info["variant"] == _abi.SPMM_EBSR and info["group_size"] >= 8 and span >= 2048
+            and 8192 % span == 0)
 
 
 @pytest.mark.parametrize("nnz", [90_000, 70_001])
