@@ -366,3 +366,23 @@ def test_permute_plan_validation(lib):
     assert lib.segscan_last_error().decode() != ""
     assert lib.segscan_permute_run(None, 0x1000, 0x2000, None) == _abi.EINVAL
     assert lib.segscan_permute_plan_destroy(None) == _abi.OK
+
+
+def test_spmm_permuted_host_validation(lib):
+    """segscan_spmm_csr_permuted refuses a missing A or plan, and an A that carries its own
+    values or value map, on the host before touching the plan or the device (the plan's own
+    checks and the launch routes: GPU tests)."""
+    d = _abi.CsrDesc()
+    d.nrow, d.ncol, d.nnz = 4, 4, 4
+    d.rowptr, d.colind = 0x1000, 0x2000
+
+    def call(desc, plan):
+        return lib.segscan_spmm_csr_permuted(desc, plan, 0x3000, 8, 0x4000, 8, _abi.F32, 0x5000, 8,
+                                            _abi.SPMM_AUTO, None, 0, None, 0, None)
+
+    assert call(None, 0x6000) == _abi.EINVAL
+    assert call(byref(d), None) == _abi.EINVAL and "plan" in lib.segscan_last_error().decode()
+    d.vals = 0x7000
+    assert call(byref(d), 0x6000) == _abi.EINVAL and "must be NULL" in lib.segscan_last_error().decode()
+    d.vals, d.vals_map = None, 0x7000
+    assert call(byref(d), 0x6000) == _abi.EINVAL and "must be NULL" in lib.segscan_last_error().decode()
