@@ -304,7 +304,7 @@ segscan_status_t segscan_invert_permutation(const int32_t* perm, int64_t n, int3
  * them (8 bytes per value + two 4-byte-per-value buffers, staging and an A-order copy for
  * segscan_spmm_csr_permuted, allocated here); it
  * synchronises `stream`. Runs of one plan must not overlap (they share the staging buffer);
- * in and out must not alias. No counterpart in the reference (it has no backward). */
+ * in and out must not alias; the current device must be the one the plan was created on. No counterpart in the reference (it has no backward). */
 typedef struct segscan_permute_plan* segscan_permute_plan_t;
 segscan_status_t segscan_permute_plan_create(const int32_t* src, int64_t n, void* stream,
                                              segscan_permute_plan_t* plan);

@@ -33,6 +33,7 @@ segscan_status_t spmm_csr_impl(const segscan_csr_t* A, int64_t N, const void* B,
 
 // A permutation plan's device buffers and passes (permute.hip).
 struct PermuteBuffers {
+  int device = 0;                   // the device whose memory holds the plan
   int64_t n = 0;
   const float* mid = nullptr;       // pass 1's output, bucket-major
   const uint16_t* slot2 = nullptr;  // [n] output position's slot inside its bucket

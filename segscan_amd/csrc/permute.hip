@@ -69,6 +69,7 @@ namespace segscan {
 
 PermuteBuffers permute_buffers(const segscan_permute_plan* plan) {
   PermuteBuffers b;
+  b.device = plan->device;
   b.n = plan->n;
   b.mid = plan->mid;
   b.slot2 = plan->slot2;
@@ -188,6 +189,10 @@ segscan_status_t segscan_permute_run(segscan_permute_plan_t plan, const float* i
   if (!plan) return einval("plan is NULL");
   if (!in || !out) return einval("in or out is NULL");
   if (in == out) return einval("in and out must not alias");
+  int device = 0;
+  hipError_t de = hipGetDevice(&device);
+  if (de != hipSuccess) return hip_fail(de, "permute run: current device");
+  if (device != plan->device) return einval("the current device is not the plan's");
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = segscan::permute_pass1(plan, in, s);
   if (e == hipSuccess) e = segscan::permute_pass2(plan, out, s);

@@ -717,6 +717,10 @@ segscan_status_t segscan_spmm_csr_permuted(const segscan_csr_t* A, segscan_permu
     return fail(SEGSCAN_EINVAL, "plan permutes %lld values, A has %lld entries", (long long)pb.n, (long long)A->nnz);
   if (!vals_src) return fail(SEGSCAN_EINVAL, "vals_src is NULL");
   if (reinterpret_cast<uintptr_t>(vals_src) % 4) return fail(SEGSCAN_EINVAL, "vals_src is not 4-byte aligned");
+  int device = 0;
+  const hipError_t de = hipGetDevice(&device);
+  if (de != hipSuccess) return hip_status(de, "current device");
+  if (device != pb.device) return fail(SEGSCAN_EINVAL, "the current device (%d) is not the plan's (%d)", device, pb.device);
   // validation and planning first: nothing is launched for a call that would fail
   segscan::StagedValues sv;
   sv.mid = pb.mid;
