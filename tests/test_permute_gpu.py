@@ -169,7 +169,8 @@ def _fuses(info) -> bool:
 def test_spmm_permuted_equals_permute_then_spmm(gpu, nnz, case):
     variant, knobs = (_FUSED + _UNFUSED + [("auto", None)])[case]
     m = synth.powerlaw_csr(M=6000, K=5000, max_deg=2000, target_nnz=nnz, seed=41 + nnz % 7)
-    for N, dtype in ((64, torch.float32), (128, torch.float32), (64, torch.bfloat16), (16, torch.float32)):
+    for N, dtype in ((64, torch.float32), (128, torch.float32), (64, torch.bfloat16), (64, torch.float16),
+                     (16, torch.float32)):
         info = _spmm_permuted_case(gpu, m, N, dtype, variant, knobs, seed=N)
         if case < len(_FUSED):
             assert _fuses(info), info
