@@ -7,7 +7,7 @@ segscan_spmm_csr_permuted (the SpMM that reads its values through a plan) equals
 permutation into a temporary followed by segscan_spmm_csr (bit for bit on integer operands,
 within 1e-6 of sum |a||b| on U(-1,1) ones): with the second pass
 fused into the SpMM (workgroup rounds of 8192, 4096 and 2048 entries, several rounds per
-workgroup, a partial last round, two column blocks, bf16 B, accumulate) and without it
+workgroup, a partial last round, two column blocks, bf16 and fp16 B, accumulate) and without it
 (launches it cannot fuse)."""
 import ctypes
 
