@@ -89,8 +89,12 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
       __syncthreads();  // every lane is done with the previous round's bucket
       const int64_t b0 = (cw * p.chunk / kPermBucket) * kPermBucket;
       const int blen = static_cast<int>(min<int64_t>(kPermBucket, p.nnz - b0));
-      const float4* src4 = reinterpret_cast<const float4*>(p.vmid + b0);
-      for (int i = L; i < (blen >> 2); i += p.tpb) reinterpret_cast<float4*>(s_bkt)[i] = src4[i];
+      // read once per workgroup: non-temporal, so the bucket does not take L2 room from the
+      // B rows the gather reuses (fwd + bwd step 0.3536 -> 0.3522 ms over five pairs)
+      typedef float f32x4 __attribute__((ext_vector_type(4)));
+      const f32x4* src4 = reinterpret_cast<const f32x4*>(p.vmid + b0);
+      for (int i = L; i < (blen >> 2); i += p.tpb)
+        reinterpret_cast<f32x4*>(s_bkt)[i] = __builtin_nontemporal_load(src4 + i);
       for (int i = (blen & ~3) + L; i < blen; i += p.tpb) s_bkt[i] = p.vmid[b0 + i];
       // the barrier that publishes the bucket follows the chunk start search
     } else if (__ballot(c < p.nchunks) == 0) {
