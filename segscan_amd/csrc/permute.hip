@@ -35,9 +35,12 @@ permute_pass1_kernel(const float* __restrict__ in, const uint16_t* __restrict__ 
   __shared__ float s[kTile];
   const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kTile;
   const int len = static_cast<int>(min<int64_t>(kTile, n - t0));
-  for (int k = threadIdx.x; k < len; k += kPermTpb) s[rank1[t0 + k]] = in[t0 + k];
+  // every stream here is read once: non-temporal loads leave L2 to the kernels around the
+  // plan (fwd + bwd step 0.3518 -> 0.3494 ms over five pairs)
+  for (int k = threadIdx.x; k < len; k += kPermTpb)
+    s[__builtin_nontemporal_load(rank1 + t0 + k)] = __builtin_nontemporal_load(in + t0 + k);
   __syncthreads();
-  for (int k = threadIdx.x; k < len; k += kPermTpb) mid[mid_pos[t0 + k]] = s[k];
+  for (int k = threadIdx.x; k < len; k += kPermTpb) mid[__builtin_nontemporal_load(mid_pos + t0 + k)] = s[k];
 }
 
 __global__ void __launch_bounds__(kPermTpb)
