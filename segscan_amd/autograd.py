@@ -9,8 +9,8 @@ over where its launch allows (segscan_spmm_csr_permuted), or, with
 forward_values="vals_map", through the inverse permutation inside the SpMM)
 and the fused backward reads them and writes their gradient coalesced, with no random
 access per entry; for CSR-order values the backward permutes them into the transpose's
-order and the gradient back with two such plans, or with backward_values="perm" reads
-and writes them through perm inside the walk):
+order and the gradient back with two such plans, or reads and writes them through perm
+inside the walk; backward_values="auto", the default, times both once per width):
     dB    = A^T . dC     CSR -> CSC transpose on the device (cached per pattern)
     dvals = SDDMM(dC, B) on A's pattern: dvals[p] = <dC[row(p)], B[colind[p]]>
     When dvals is wanted, both come from one fused walk over the transpose (one
@@ -33,13 +33,13 @@ class SpmmPattern:
 
     def __init__(self, A: ops.CsrMatrix, variant="auto", knobs=None, variant_t="auto", knobs_t=None,
                  fused: bool = True, knobs_bwd=None, values_order: str = "csr", forward_values: str = "permute",
-                 backward_values: str = "permute"):
+                 backward_values: str = "auto"):
         if values_order not in ("csr", "transpose"):
             raise ValueError("values_order must be 'csr' or 'transpose'")
         if forward_values not in ("permute", "vals_map"):
             raise ValueError("forward_values must be 'permute' or 'vals_map'")
-        if backward_values not in ("permute", "perm"):
-            raise ValueError("backward_values must be 'permute' or 'perm'")
+        if backward_values not in ("auto", "permute", "perm"):
+            raise ValueError("backward_values must be 'auto', 'permute' or 'perm'")
         self.A = A
         self.values_order = values_order
         T = ops.csr_transpose(A)
@@ -56,20 +56,53 @@ class SpmmPattern:
         self.forward_values = forward_values
         self.permute = (ops.PermutePlan(self.iperm) if values_order == "transpose" and forward_values == "permute"
                         and A.nnz > 0 else None)
-        # values in A's order: the fused backward takes them into the transpose's order and
-        # its dvals back with two permutation plans (streaming passes) instead of one random
-        # read and one random write per entry inside the walk; built on first use
+        # values in A's order: the fused backward either takes them into the transpose's order
+        # and its dvals back with two permutation plans (streaming passes, "permute") or reads
+        # and writes them through perm inside the walk (one random 4-byte read and write per
+        # entry, "perm"). Which is faster depends on the matrix (power-law 262144^2, N = 64:
+        # 0.399 against 0.429 ms per fwd + bwd; the GCN graph, 131072 nodes: 0.267 against
+        # 0.240 ms), so "auto" times both on the first backward of each width and B type and
+        # keeps the faster. Both run the same walk over the same values: dvals are equal bit
+        # for bit, and dB differs at most as two runs of either do (the order in which the
+        # carries of rows that span chunks are added). Plans are built on first use.
         self.backward_values = backward_values
         self._to_t = self._to_csr = None
+        self.backward_choice = {}  # (N, B dtype, dB wanted) -> "permute" | "perm" ("auto")
 
     def csr_permute_plans(self):
         """(to the transpose's order, back to A's order) permutation plans, or None."""
-        if self.values_order != "csr" or self.backward_values != "permute" or self.A.nnz == 0:
+        if self.values_order != "csr" or self.backward_values == "perm" or self.A.nnz == 0:
             return None
         if self._to_t is None:
             self._to_t = ops.PermutePlan(self.T.perm)  # x_t[q] = x[perm[q]]
             self._to_csr = ops.PermutePlan(ops.invert_permutation(self.T.perm))  # x[p] = x_t[iperm[p]]
         return self._to_t, self._to_csr
+
+    def csr_backward_mode(self, key, via_plans, in_walk) -> str:
+        """The backward route for values in A's order: the fixed choice, or for "auto" the
+        faster of the two on their first use for `key` (each run once to warm up, then three
+        times between HIP events on the current stream). Under hipGraph capture nothing is
+        timed and "permute" runs."""
+        if self.backward_values != "auto":
+            return self.backward_values
+        if self.csr_permute_plans() is None:
+            return "perm"
+        if key in self.backward_choice:
+            return self.backward_choice[key]
+        if torch.cuda.is_current_stream_capturing():
+            return "permute"
+        ms = {}
+        for mode, fn in (("permute", via_plans), ("perm", in_walk)):
+            fn()
+            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record()
+            for _ in range(3):
+                fn()
+            stop.record()
+            stop.synchronize()
+            ms[mode] = start.elapsed_time(stop)
+        self.backward_choice[key] = min(ms, key=ms.get)
+        return self.backward_choice[key]
 
     def fused_ok(self, N: int) -> bool:
         """Whether the fused backward covers N columns (one group of lanes per dense row)."""
@@ -148,15 +181,20 @@ class SpmmFunction(torch.autograd.Function):
                 # values and their gradient in the walk's own order: no perm at all
                 dB, dvals = ops.spmm_backward(pattern.T, None, dC, Bs, want_dB=want_b, knobs=pattern.knobs_bwd,
                                               vals_t=v32, dvals_at_order=True)
-            elif pattern.csr_permute_plans() is not None:
-                # values taken into the walk's order and dvals back by two streaming passes
-                to_t, to_csr = pattern.csr_permute_plans()
-                dB, dvals_t = ops.spmm_backward(pattern.T, None, dC, Bs, want_dB=want_b, knobs=pattern.knobs_bwd,
-                                                vals_t=to_t(v32), dvals_at_order=True)
-                dvals = to_csr(dvals_t)
             else:
-                # values read through the permutation inside the kernel
-                dB, dvals = ops.spmm_backward(pattern.T, v32, dC, Bs, want_dB=want_b, knobs=pattern.knobs_bwd)
+                def via_plans():
+                    # values taken into the walk's order and dvals back by two streaming passes
+                    to_t, to_csr = pattern.csr_permute_plans()
+                    gB, dvals_t = ops.spmm_backward(pattern.T, None, dC, Bs, want_dB=want_b,
+                                                    knobs=pattern.knobs_bwd, vals_t=to_t(v32), dvals_at_order=True)
+                    return gB, to_csr(dvals_t)
+
+                def in_walk():
+                    # values read and dvals written through the permutation inside the kernel
+                    return ops.spmm_backward(pattern.T, v32, dC, Bs, want_dB=want_b, knobs=pattern.knobs_bwd)
+
+                mode = pattern.csr_backward_mode((dC.size(1), Bs.dtype, want_b), via_plans, in_walk)
+                dB, dvals = via_plans() if mode == "permute" else in_walk()
             return dvals.to(vals.dtype), (None if dB is None else dB.to(B.dtype)), None
         # dB alone (fixed values): the SpMM kernels on the transpose, values in its order (cached)
         if ctx.needs_input_grad[1]:

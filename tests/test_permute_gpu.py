@@ -82,24 +82,32 @@ def test_autograd_forward_values_modes_agree(gpu):
 
 
 def test_autograd_csr_order_backward_modes_agree(gpu):
-    """Values in A's order: the backward through two permutation plans (the default) and
-    through perm inside the walk give the same gradients (integer operands: exact)."""
+    """Values in A's order: the backward through two permutation plans, through perm inside
+    the walk, and "auto" (the default: both timed on the first backward, the faster kept for
+    later ones) give the same gradients (integer operands: exact)."""
     m = synth.powerlaw_csr(M=6000, K=5000, max_deg=2000, target_nnz=90_000, seed=33, values="int")
     A = m.to_device(gpu)
     rng = np.random.default_rng(34)
     B = torch.from_numpy(rng.integers(-2, 3, size=(m.ncol, 64)).astype(np.float32)).to(gpu)
     dC = torch.from_numpy(rng.integers(-2, 3, size=(m.nrow, 64)).astype(np.float32)).to(gpu)
     outs = []
-    for mode in ("permute", "perm"):
+    for mode in ("permute", "perm", "auto"):
         pat = autograd.SpmmPattern(A, backward_values=mode)
-        assert (pat.csr_permute_plans() is not None) == (mode == "permute")
-        vals = A.vals.clone().requires_grad_(True)
-        Bp = B.clone().requires_grad_(True)
-        C = autograd.spmm(vals, Bp, pat)
-        C.backward(dC)
-        outs.append((C.detach(), vals.grad, Bp.grad))
-    for a, b in zip(outs[0], outs[1]):
-        assert torch.equal(a, b)
+        assert (pat.csr_permute_plans() is not None) == (mode != "perm")
+        for _ in range(2 if mode == "auto" else 1):  # auto: the timed first call, then the kept route
+            vals = A.vals.clone().requires_grad_(True)
+            Bp = B.clone().requires_grad_(True)
+            C = autograd.spmm(vals, Bp, pat)
+            C.backward(dC)
+            outs.append((C.detach(), vals.grad, Bp.grad))
+        if mode == "auto":
+            assert list(pat.backward_choice.values()) in (["permute"], ["perm"])
+        else:
+            assert pat.backward_choice == {}
+    assert autograd.SpmmPattern(A).backward_values == "auto"
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert torch.equal(a, b)
 
 
 def _spmm_permuted_case(gpu, m, N, dtype, variant, knobs, accumulate=False, seed=0):
