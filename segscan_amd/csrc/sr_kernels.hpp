@@ -33,9 +33,14 @@
 namespace segscan {
 
 // VB: values from a permutation plan's staged buckets (SrParams::vslot); SpMM only.
-template <typename VT, int CF, int G, bool SEG, bool VB = false>
+// OWN: the owner write policy without accumulation, fixed at compile time (the tuned
+// points' case): the flush path then holds one store form (tuned fp32 kernel 167.7 -> 166.4
+// us, bf16 0.1156 -> 0.1141 ms per call, fwd + bwd step 0.3494 -> 0.3472 ms, three pairs);
+// SpMM only.
+template <typename VT, int CF, int G, bool SEG, bool VB = false, bool OWN = false>
 __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   static_assert(!(SEG && VB), "staged values are an SpMM mode");
+  static_assert(!(SEG && OWN), "the fixed owner form is an SpMM mode");
   extern __shared__ int32_t sr_smem[];
   int32_t* s_k = sr_smem;                                  // [tpb] column (SpMM)
   float* s_v = reinterpret_cast<float*>(sr_smem + p.tpb);  // [tpb] value (SpMM)
@@ -71,8 +76,8 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
   // Every lane of the wave holds CF columns: full tiles can skip per-lane column checks.
   const bool lanes_full = __ballot(!full) == 0;
   const int sbase = g * G;  // this group's LDS slots
-  const bool acc_mode = p.accumulate != 0;
-  const bool owner = p.policy == 2;
+  const bool acc_mode = !OWN && p.accumulate != 0;
+  const bool owner = OWN || p.policy == 2;
   // empty runs: rows zeroed here, whole blocks flagged for the fix-up pass
   // (segments flag entry-free blocks; for a CSR matrix the fix-up pass reads rowptr)
   const bool zrun = owner && !acc_mode;
@@ -148,7 +153,7 @@ __global__ void __launch_bounds__(1024) sr_kernel(SrParams p) {
       if (owner) {
         if (row == first_row && is_carry) store_cols<CF>(p.carry_val + c * p.N + col0, nv, acc, false);
         else store_cols<CF>(dst, nv, acc, acc_mode);
-      } else if (p.policy == 3 && !last && row != first_row) {
+      } else if (!OWN && p.policy == 3 && !last && row != first_row) {
         store_cols<CF>(dst, nv, acc, acc_mode);
       } else {
         atomic_add_cols<CF>(dst, nv, acc);
@@ -465,18 +470,28 @@ __global__ void __launch_bounds__(256) sddmm_sr_kernel(SddmmParams p) {
 
 // Staged-value launches (p.vslot set) exist for groups of 8 to 64 lanes; the host plans
 // them only there.
+// The fixed owner form (owner policy, no accumulation) exists for groups of 8 to 64 lanes too;
+// other groups take the general kernel for it.
 template <typename VT, int CF, int G>
 static hipError_t launch_sr_t(const SrParams& p, const Launch& l, bool seg, bool& ok) {
+  const bool own = p.policy == 2 && p.accumulate == 0;
   if (seg) {
     sr_kernel<VT, CF, G, true><<<l.grid, l.block, l.shmem, l.stream>>>(p);
   } else if (p.vslot) {
     if constexpr (G >= 8) {
-      sr_kernel<VT, CF, G, false, true><<<l.grid, l.block, l.shmem, l.stream>>>(p);
+      if (own) sr_kernel<VT, CF, G, false, true, true><<<l.grid, l.block, l.shmem, l.stream>>>(p);
+      else sr_kernel<VT, CF, G, false, true><<<l.grid, l.block, l.shmem, l.stream>>>(p);
     } else {
       ok = false;
       return hipSuccess;
     }
   } else {
+    if constexpr (G >= 8) {
+      if (own) {
+        sr_kernel<VT, CF, G, false, false, true><<<l.grid, l.block, l.shmem, l.stream>>>(p);
+        return hipGetLastError();
+      }
+    }
     sr_kernel<VT, CF, G, false><<<l.grid, l.block, l.shmem, l.stream>>>(p);
   }
   return hipGetLastError();
